@@ -1,0 +1,163 @@
+#!/usr/bin/env python3
+"""Throughput of the D3Q15 lattice-Boltzmann step on one MI355X.
+
+Workload (default): the porous-media permeability case -- a periodic 256^3
+box filled by random sequential addition of non-overlapping spheres of
+radius 8 (seeded splitmix64, minimum-image overlap test, periodic images
+inserted) to a lattice solid fraction of 0.35, flattened to one object,
+uniform body force 1e-6 along x on fluid nodes, Stokes flow, tau 0.75, FP32
+shifted populations.  One step = one fused bounce-back + stream + collide
+pass over all 256^3 nodes (MLUPS counts every node, fluid and solid, as the
+reference's ksite-updates/sec does).
+
+Prints one JSON line.  The timed region is K steps between HIP events on
+the lattice's stream with the populations already resident in HBM.
+The cpu_baseline leg times the reference library (oracle/_ref, built from
+the reference's own sources, -D_SINGLE) single-threaded on the same
+geometry for a few steps.
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_HBM_GBS = 8000.0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def build_pack(L, n, R, phi, seed, force):
+    h = L.newlb(n, n, n, b"null")
+    s = h.contents
+    s.stokes_flow = 1
+    s.tau = 0.75
+    phi_out = C.c_double(0.0)
+    t0 = time.perf_counter()
+    nsph = L.slb_rsa_pack(h, R, phi, seed, 50_000_000, None, 0, C.byref(phi_out))
+    L.flattenobj(h)
+    L.zerobf(h)
+    L.addbf(h, force, 0.0, 0.0)
+    L.syncobjs(h)
+    links = L.countmask(h, b"links")
+    log(f"pack: {nsph} spheres, lattice solid fraction {phi_out.value:.4f}, "
+        f"weighted links {links}, setup {time.perf_counter() - t0:.1f} s")
+    return h, nsph, phi_out.value
+
+
+def cpu_baseline(L, h, n, force, steps):
+    """Time the reference library (FP32 build) on the same solid map."""
+    from oracle import RefLib
+    try:
+        R = RefLib(32)
+    except ImportError as e:
+        log(f"cpu_baseline: reference library unavailable ({e})")
+        return None
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "map.dat")
+        L.lbsavemap(h, path.encode(), b"char")
+        rh = R.L.newlb(n, n, n, b"null")
+        rs = rh.contents
+        rs.stokes_flow = 1
+        rs.tau = 0.75
+        R.L.insertsubs(rh, path.encode(), 0, n - 1, 0, n - 1, 0, n - 1)
+    R.L.zerobf(rh)
+    R.L.addbf(rh, force, 0.0, 0.0)
+    R.L.lbstep(rh)  # first step syncs the tables
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        R.L.lbstep(rh)
+    dt = time.perf_counter() - t0
+    mlups = n ** 3 * steps / dt / 1e6
+    return dict(value=round(mlups, 3), unit="MLUPS", cores=1, kind="reference",
+                sample=f"reference SunlightLB (-D_SINGLE, gcc -O3 -ffast-math, serial) on the "
+                       f"same {n}^3 pack, {steps} lbstep calls after one untimed step, "
+                       f"{dt:.1f} s")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--size", type=int, default=256)
+    ap.add_argument("--radius", type=float, default=8.0)
+    ap.add_argument("--phi", type=float, default=0.35)
+    ap.add_argument("--seed", type=int, default=20261202)
+    ap.add_argument("--force", type=float, default=1e-6)
+    ap.add_argument("--precision", type=int, default=32, choices=(32, 64))
+    ap.add_argument("--cpu-steps", type=int, default=4)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py runs on exactly one GPU: the lattice-Boltzmann path here is "
+              "single-GPU (multi-GPU slab decomposition is designed, not built).",
+              file=sys.stderr)
+        sys.exit(2)
+
+    os.environ["SLB_PRECISION"] = str(args.precision)
+    import sunlightlb_amd as s
+    L = s.lib()
+    n = args.size
+    h, nsph, phi = build_pack(L, n, args.radius, args.phi, args.seed, args.force)
+    N = n ** 3
+
+    L.slb_run(h, args.warmup)
+    ms = L.slb_time_steps(h, args.steps)
+    ms_per_step = ms / args.steps
+    mlups = N * args.steps / (ms * 1e-3) / 1e6
+    bytes_per_lup = 2 * 15 * (4 if args.precision == 32 else 8)
+    achieved = bytes_per_lup * N / (ms_per_step * 1e-3) / 1e9
+    log(f"gpu: {ms_per_step * 1e3:.1f} us/step, {mlups:.1f} MLUPS, {achieved:.0f} GB/s "
+        f"algorithmic ({bytes_per_lup} B/LUP)")
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(L, h, n, args.force, args.cpu_steps)
+
+    dtype = "f32" if args.precision == 32 else "f64"
+    out = {
+        "metric": "MLUPS (D3Q15 fused bounce-back+stream+collide, all nodes)",
+        "value": round(mlups, 2),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 6),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": dtype,
+        "data": "synthetic",
+        "config": {
+            "workload": f"porous-media permeability: periodic {n}^3 RSA pack of radius-"
+                        f"{args.radius:g} spheres to lattice solid fraction {args.phi:g} "
+                        f"(seed {args.seed}), body force {args.force:g} x, Stokes, tau 0.75",
+            "nodes": N, "spheres": nsph, "solid_fraction": round(phi, 5),
+            "populations": f"{dtype} SoA shifted (a = f - w rho0/72), pull, ping-pong",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(achieved, 1),
+            "peak": PEAK_HBM_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / PEAK_HBM_GBS, 4),
+            "traffic": None,
+            "kernel": "k_step (fused pull step)",
+            "algorithmic_bytes_per_node": bytes_per_lup,
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

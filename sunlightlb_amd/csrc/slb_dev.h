@@ -1,0 +1,126 @@
+/*
+ * slb_dev.h -- internal C ABI between the C host layer (slb_*.c) and the
+ * gfx950 HIP kernels (slb_dev.hip).  Plain pointers and sizes only.
+ *
+ * Device data layout for one lattice of N = esi*esj*esk nodes
+ * (node index n = esk*(esj*i+j)+k, k fastest):
+ *   pops[2]   structure-of-arrays populations, pops[b][l*N + n], T = float or
+ *             double (the run-time precision); ping-pong pair for the pull
+ *             step, `cur` names the live one (post-collide between steps)
+ *   flags     uint16 per node: bit l-1 (l=1..14) = the population arriving
+ *             along c_l is bounced back (link with exactly one solid end),
+ *             bit 14 = node is solid
+ *   objidx    int32 per node, -1 fluid, else dense object index
+ *   mask      int32 per node, the reference boundary-link mask (bits 0..13)
+ *             plus the adsorption bit 0x4000
+ *   ux,uy,uz  double per node: wall velocity (rho0/12 scaled) on solid nodes,
+ *             vgrab accumulator on fluid nodes
+ *   fx,fy,fz  double per node, momentum-exchange force (allocated lazily)
+ *   bx,by,bz  T per node, body force (only when it is not uniform)
+ */
+#ifndef SLB_DEV_H
+#define SLB_DEV_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct slbd_lattice slbd_lattice;
+
+/* Per-call physics parameters, read from the host struct at every call. */
+typedef struct {
+  double tau, rho0;
+  int stokes_flow, skip_solid, moving;
+  int bf_mode;          /* 0: no body force, 1: uniform bf[], 2: per-node arrays + bf[] */
+  double bf[3];         /* uniform part of the body force on fluid nodes */
+} slbd_params;
+
+/* Object table row: pos[3], vel[3], ang[3]. */
+#define SLBD_OBJ_STRIDE 9
+
+int  slbd_device_count(void);
+slbd_lattice *slbd_create(int esi, int esj, int esk, int prec);
+void slbd_destroy(slbd_lattice *L);
+int  slbd_precision(const slbd_lattice *L);
+int  slbd_set_precision(slbd_lattice *L, int prec, double *scratch_aos);
+void slbd_sync(slbd_lattice *L);
+const char *slbd_arch(void);
+
+/* host <-> device transfers (host populations are AoS double a[15n+l]) */
+void slbd_put_pops(slbd_lattice *L, const double *aos);
+void slbd_get_pops(slbd_lattice *L, double *aos);
+void slbd_zero_pops(slbd_lattice *L);
+void slbd_put_objidx(slbd_lattice *L, const int32_t *objidx);
+void slbd_get_objidx(slbd_lattice *L, int32_t *objidx);
+void slbd_makemask(slbd_lattice *L);                 /* mask from objidx (bit-exact) */
+void slbd_put_mask(slbd_lattice *L, const int32_t *mask);
+void slbd_get_mask(slbd_lattice *L, int32_t *mask);
+void slbd_build_flags(slbd_lattice *L);              /* pull flags from mask + objidx */
+void slbd_put_u(slbd_lattice *L, const double *ux, const double *uy, const double *uz);
+void slbd_get_u(slbd_lattice *L, double *ux, double *uy, double *uz);
+void slbd_put_bf(slbd_lattice *L, const double *bx, const double *by, const double *bz);
+void slbd_get_f(slbd_lattice *L, double *fx, double *fy, double *fz);
+void slbd_put_f(slbd_lattice *L, const double *fx, const double *fy, const double *fz);
+void slbd_zero_f(slbd_lattice *L);
+void slbd_scale_f(slbd_lattice *L, double s);
+void slbd_scale_pops(slbd_lattice *L, double s);
+
+/* LB step: nsteps fused bbl+propagate+collide; force != 0 accumulates the
+ * bblforce node force (fx,fy,fz) during the bounce-back. */
+void slbd_step(slbd_lattice *L, const slbd_params *P, int nsteps, int force);
+/* Separate phases on the live buffer. */
+void slbd_bbl(slbd_lattice *L, const slbd_params *P, int force);
+void slbd_propagate(slbd_lattice *L);
+void slbd_collide(slbd_lattice *L, const slbd_params *P);
+
+/* Reductions (FP64). out[0..2] = sum over fluid of w (j + fac b)/rho,
+ * out[3] = sum w, out[4] = fluid count.  psi may be NULL (weights 1). */
+void slbd_vmean(slbd_lattice *L, const slbd_params *P, double fac,
+                const double *psi_host, double out[5]);
+/* ux += prefac (j + fac b)/rho on fluid nodes (vgrab); vzero zeroes fluid. */
+void slbd_vgrab(slbd_lattice *L, const slbd_params *P, double fac, double prefac);
+void slbd_vzero(slbd_lattice *L);
+void slbd_vspread(slbd_lattice *L, int nobj, const double *objtab, double rho0);
+void slbd_distribf(slbd_lattice *L, int nobj, const double *objtab,
+                   double *frc, double *trq);
+/* Relative velocity residual ||u - u_prev|| / ||u|| over fluid nodes, using
+ * the corrected post-collide velocity; stores u as the next u_prev.  Returns
+ * -1 on the first call (no previous field). */
+double slbd_residual(slbd_lattice *L, const slbd_params *P);
+void slbd_residual_reset(slbd_lattice *L);
+/* Sum over fluid nodes of the 10 moments (checks / conservation tests). */
+void slbd_sum_pops(slbd_lattice *L, double out[15]);
+
+/* Moment propagation.  The frozen density and the MP link flags belong to
+ * the lattice; each MP system owns its fields (P, and with nmom = 3 the
+ * displacement moments m1, m2), ping-pong pairs in the lattice precision. */
+typedef struct slbd_mp slbd_mp;
+void slbd_mp_rhoinit(slbd_lattice *L, double rho0);
+void slbd_mp_put_rho(slbd_lattice *L, const double *rho);
+void slbd_mp_get_rho(slbd_lattice *L, double *rho);
+slbd_mp *slbd_mp_create(slbd_lattice *L, int nmom);
+void slbd_mp_destroy(slbd_mp *M);
+void slbd_mp_put(slbd_lattice *L, slbd_mp *M, int field, const double *v);
+void slbd_mp_get(slbd_lattice *L, slbd_mp *M, int field, double *v);
+void slbd_mp_zero(slbd_lattice *L, slbd_mp *M, int field);
+void slbd_mp_step(slbd_lattice *L, slbd_mp *M, double rho0, double alpha, int moving,
+                  int nobj, const double *objtab, double pads, int nmom,
+                  int axis, int nsteps);
+/* out: [0] sum fluid P, [1] sum solid P, [2] sum adsorbing-site P,
+ *      [3] sum fluid m1, [4] sum fluid m2, [5] fluid count */
+void slbd_mp_sums(slbd_lattice *L, slbd_mp *M, double out[6]);
+
+/* Timing support: run nsteps fused steps between HIP events on the
+ * lattice stream; returns the elapsed milliseconds. */
+double slbd_time_steps(slbd_lattice *L, const slbd_params *P, int nsteps);
+double slbd_time_mp(slbd_lattice *L, slbd_mp *M, double rho0, double alpha,
+                    double pads, int nmom, int nsteps);
+size_t slbd_bytes(const slbd_lattice *L);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

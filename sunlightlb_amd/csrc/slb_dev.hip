@@ -1,0 +1,1558 @@
+/*
+ * slb_dev.hip -- gfx950 kernels for the SunlightLB D3Q15 lattice-Boltzmann
+ * step and the thin C ABI (slb_dev.h) the C host layer calls.
+ *
+ * The hot path is one fused pull kernel per time step (k_step): every node
+ * gathers the 15 post-collide populations that stream into it, applies link
+ * bounce-back where the link has exactly one solid end, collides, and writes
+ * its 15 new populations to the other buffer of a ping-pong pair.  This is
+ * bbl -> propagate -> collide of src/lbkernel.c:49-51 in one pass over HBM:
+ * each population is read once and written once.
+ *
+ * Equivalence of the pull form with the reference's swap-then-shift
+ * (src/lbkernel.c:58-618 then :623-762): after the reference's bbl and
+ * propagate, the value at x along l is
+ *     a_lbar(x) + w_l c_l.U(s)   if exactly one of x, x - c_l is solid
+ *                                (s = the solid one; U from vspread),
+ *     a_l(x - c_l)               otherwise,
+ * with all a post-collide.  The per-node `flags` word holds the first case
+ * as one bit per link, derived from the reference mask (bit l-1 of the solid
+ * node x - c_l, or bit lbar-1 of x itself).
+ *
+ * Momentum-exchange force (bblforce, src/lbkernel.c:179-321, :464-618): for a
+ * solid node s with boundary link m (f = s + c_m fluid) the reference adds
+ * 2 (a_m(s) - a_mbar(f) - w_m c_m.U(s)) c_m to the force of s, using values
+ * before the swap.  In the pull kernel the solid node s sees that link as an
+ * incoming bounce along l = mbar; it already loads a_m(s) and loads a_mbar(f)
+ * = a_l(s - c_l) as well, so each solid node accumulates its own force with
+ * no atomics.
+ */
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+#include <type_traits>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include "slb_dev.h"
+
+#define HIPCHECK(call)                                                        \
+  do {                                                                        \
+    hipError_t e_ = (call);                                                   \
+    if (e_ != hipSuccess) {                                                   \
+      fprintf(stderr, "sunlightlb: HIP error '%s' (%d) at %s:%d in %s\n",     \
+              hipGetErrorString(e_), (int)e_, __FILE__, __LINE__, #call);     \
+      fflush(stderr);                                                         \
+      exit(EXIT_FAILURE);                                                     \
+    }                                                                         \
+  } while (0)
+
+#define LAUNCHCHECK() HIPCHECK(hipGetLastError())
+
+/* ---------------------------------------------------------------------- */
+/* D3Q15 tables (the reference's src/lbmisc.c:89-104 ordering).            */
+
+__host__ __device__ constexpr int CX(int l) {
+  return (l == 1 || l == 7 || l == 9 || l == 11 || l == 13) ? 1
+       : (l == 2 || l == 8 || l == 10 || l == 12 || l == 14) ? -1 : 0;
+}
+__host__ __device__ constexpr int CY(int l) {
+  return (l == 3 || l == 7 || l == 8 || l == 11 || l == 12) ? 1
+       : (l == 4 || l == 9 || l == 10 || l == 13 || l == 14) ? -1 : 0;
+}
+__host__ __device__ constexpr int CZ(int l) {
+  return (l == 5 || (l >= 7 && l <= 10)) ? 1 : (l == 6 || l >= 11) ? -1 : 0;
+}
+__host__ __device__ constexpr int WL(int l) { return l == 0 ? 16 : (l <= 6 ? 8 : 1); }
+__host__ __device__ constexpr int OPP(int l) {
+  return l == 0 ? 0 : (l <= 6 ? ((l & 1) ? l + 1 : l - 1) : 21 - l);
+}
+
+#define FLAG_SOLID 0x4000u
+#define FLAG_LINKS 0x3fffu
+
+/* Neighbour offsets along one axis for the node at coordinate q of an axis of
+ * length n and stride st: off[0] for component -1 (source q+1), off[2] for
+ * component +1 (source q-1); used with sign = -1 for pull (x - c) or +1
+ * for push (x + c). */
+struct AxisOff {
+  long m, p;  /* offset of q-1 and q+1 with periodic wrap */
+};
+__device__ __forceinline__ AxisOff axis_off(int q, int n, long st) {
+  AxisOff o;
+  o.m = (q > 0) ? -st : (long)(n - 1) * st;
+  o.p = (q < n - 1) ? st : -(long)(n - 1) * st;
+  return o;
+}
+/* offset of x + s*c_l (s = +1 push, -1 pull) */
+template <int L, int S>
+__device__ __forceinline__ long nb_off(const AxisOff &oi, const AxisOff &oj, const AxisOff &ok) {
+  constexpr int cx = S * CX(L), cy = S * CY(L), cz = S * CZ(L);
+  long o = 0;
+  if (cx > 0) o += oi.p; else if (cx < 0) o += oi.m;
+  if (cy > 0) o += oj.p; else if (cy < 0) o += oj.m;
+  if (cz > 0) o += ok.p; else if (cz < 0) o += ok.m;
+  return o;
+}
+
+struct Geom {
+  int esi, esj, esk, plane;
+  long N;
+};
+
+/* Compile-time unrolled loop over links 1..14 */
+template <int L, int END>
+struct Links {
+  template <typename F>
+  __device__ __forceinline__ static void run(F &&f) {
+    f(std::integral_constant<int, L>{});
+    Links<L + 1, END>::run(f);
+  }
+};
+template <int END>
+struct Links<END, END> {
+  template <typename F>
+  __device__ __forceinline__ static void run(F &&) {}
+};
+
+/* ---------------------------------------------------------------------- */
+/* Collision of one node (src/lbkernel.c:797-866).                        */
+
+struct PhysArgs {
+  double omega, comega, rho0;
+  int stokes, skip_solid;
+  double bfx, bfy, bfz;
+};
+
+template <typename T>
+__device__ __forceinline__ void collide_node(T v[15], T jbx, T jby, T jbz,
+                                             const PhysArgs &P) {
+  const T ax = v[1] + v[2], dx = v[1] - v[2];
+  const T ay = v[3] + v[4], dy = v[3] - v[4];
+  const T az = v[5] + v[6], dz = v[5] - v[6];
+  /* diagonal links 7..14, c = (sx,sy,sz): sums weighted by 1, sx, sy, sz,
+   * sx sy, sx sz, sy sz */
+  const T p7 = v[7] + v[14], m7 = v[7] - v[14];     /* (+,+,+) */
+  const T p8 = v[8] + v[13], m8 = v[8] - v[13];     /* (-,+,+) */
+  const T p9 = v[9] + v[12], m9 = v[9] - v[12];     /* (+,-,+) */
+  const T p10 = v[10] + v[11], m10 = v[10] - v[11]; /* (-,-,+) */
+  const T D = (p7 + p8) + (p9 + p10);
+  const T Dx = (m7 - m8) + (m9 - m10);
+  const T Dy = (m7 + m8) - (m9 + m10);
+  const T Dz = (m7 + m8) + (m9 + m10);
+  const T Dxy = (p7 - p8) - (p9 - p10);
+  const T Dxz = (p7 - p8) + (p9 - p10);
+  const T Dyz = (p7 + p8) - (p9 + p10);
+  const T a0a = v[0] + ax + ay + az;
+  const T drho = a0a + D;
+  const T jx = dx + Dx + jbx;
+  const T jy = dy + Dy + jby;
+  const T jz = dz + Dz + jbz;
+  /* traceless-shifted stress s = Pi - drho/3 I (src/lbkernel.c:821-823) */
+  const T sh = T(2.0 / 3.0) * drho - a0a;
+  T sxx = sh + ax, syy = sh + ay, szz = sh + az;
+  T sxy = Dxy, sxz = Dxz, syz = Dyz;
+  const T cw = T(P.comega);
+  if (P.stokes) {
+    sxx *= cw; syy *= cw; szz *= cw; sxy *= cw; sxz *= cw; syz *= cw;
+  } else {
+    const T o = T(P.omega) / (T(P.rho0) + drho);
+    sxx = cw * sxx + o * jx * jx;
+    syy = cw * syy + o * jy * jy;
+    szz = cw * szz + o * jz * jz;
+    sxy = cw * sxy + o * jx * jy;
+    sxz = cw * sxz + o * jx * jz;
+    syz = cw * syz + o * jy * jz;
+  }
+  /* rebuild a_l = (w_l/72)[drho + 3 c.j + 4.5 (cc:s - tr s/3)] */
+  const T r72 = drho * T(1.0 / 72.0);
+  const T r9 = T(8.0) * r72;
+  const T t3 = (sxx + syy + szz) * T(1.0 / 3.0);
+  const T third = T(1.0 / 3.0), half = T(0.5), eighth = T(0.125), d24 = T(1.0 / 24.0);
+  v[0] = T(2.0) * r9 - t3;
+  T q, g;
+  q = r9 + half * (sxx - t3); g = third * jx; v[1] = q + g; v[2] = q - g;
+  q = r9 + half * (syy - t3); g = third * jy; v[3] = q + g; v[4] = q - g;
+  q = r9 + half * (szz - t3); g = third * jz; v[5] = q + g; v[6] = q - g;
+  q = r72 + eighth * (t3 + sxy + sxz + syz); g = d24 * (jx + jy + jz);
+  v[7] = q + g; v[14] = q - g;
+  q = r72 + eighth * (t3 - sxy - sxz + syz); g = d24 * (-jx + jy + jz);
+  v[8] = q + g; v[13] = q - g;
+  q = r72 + eighth * (t3 - sxy + sxz - syz); g = d24 * (jx - jy + jz);
+  v[9] = q + g; v[12] = q - g;
+  q = r72 + eighth * (t3 + sxy - sxz - syz); g = d24 * (-jx - jy + jz);
+  v[10] = q + g; v[11] = q - g;
+}
+
+/* ---------------------------------------------------------------------- */
+/* Fused step kernel.                                                      */
+
+template <typename T, bool MOVING, bool FORCE, int BF>
+__global__ void __launch_bounds__(256)
+k_step(const T *__restrict__ src, T *__restrict__ dst,
+       const uint16_t *__restrict__ flags,
+       const double *__restrict__ ux, const double *__restrict__ uy,
+       const double *__restrict__ uz,
+       const T *__restrict__ bx, const T *__restrict__ by, const T *__restrict__ bz,
+       double *__restrict__ fx, double *__restrict__ fy, double *__restrict__ fz,
+       Geom G, PhysArgs P) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= G.plane) return;
+  const int i = blockIdx.y;
+  const int j = t / G.esk;
+  const int k = t - j * G.esk;
+  const long N = G.N;
+  const long x = (long)i * G.plane + t;
+  const AxisOff oi = axis_off(i, G.esi, G.plane);
+  const AxisOff oj = axis_off(j, G.esj, G.esk);
+  const AxisOff ok = axis_off(k, G.esk, 1);
+  const unsigned fl = flags[x];
+  const bool solid = (fl & FLAG_SOLID) != 0;
+  T v[15];
+  v[0] = src[x];
+  Links<1, 15>::run([&](auto LL) {
+    constexpr int l = decltype(LL)::value;
+    const bool bb = (fl >> (l - 1)) & 1u;
+    const long idx = bb ? (long)OPP(l) * N + x : (long)l * N + x + nb_off<l, -1>(oi, oj, ok);
+    v[l] = src[idx];
+  });
+  if ((MOVING || FORCE) && (fl & FLAG_LINKS)) {
+    double Fx = 0.0, Fy = 0.0, Fz = 0.0;
+    Links<1, 15>::run([&](auto LL) {
+      constexpr int l = decltype(LL)::value;
+      if ((fl >> (l - 1)) & 1u) {
+        const long xs = x + nb_off<l, -1>(oi, oj, ok);
+        double cu = 0.0;
+        if (MOVING) {
+          const long s = solid ? x : xs;
+          cu = (double)WL(l) * ((CX(l) ? CX(l) * ux[s] : 0.0) + (CY(l) ? CY(l) * uy[s] : 0.0) +
+                                (CZ(l) ? CZ(l) * uz[s] : 0.0));
+        }
+        if (FORCE && solid) {
+          /* link m = lbar of solid x: a = a_m(x), b = a_l(x - c_l) */
+          const double a = (double)v[l];
+          const double b = (double)src[(long)l * N + xs];
+          const double g = -2.0 * (a - b + cu);
+          if (CX(l)) Fx += CX(l) * g;
+          if (CY(l)) Fy += CY(l) * g;
+          if (CZ(l)) Fz += CZ(l) * g;
+        }
+        if (MOVING) v[l] = (T)((double)v[l] + cu);
+      }
+    });
+    if (FORCE && solid) {
+      fx[x] += Fx; fy[x] += Fy; fz[x] += Fz;
+    }
+  }
+  if (!(P.skip_solid && solid)) {
+    T jbx = T(0), jby = T(0), jbz = T(0);
+    if (BF != 0 && !solid) {
+      jbx = T(P.bfx); jby = T(P.bfy); jbz = T(P.bfz);
+      if (BF == 2) { jbx += bx[x]; jby += by[x]; jbz += bz[x]; }
+    }
+    collide_node<T>(v, jbx, jby, jbz, P);
+  }
+#pragma unroll
+  for (int l = 0; l < 15; l++) dst[(long)l * N + x] = v[l];
+}
+
+/* ---------------------------------------------------------------------- */
+/* Separate phases (bbl, propagate, collide) on the live buffer.           */
+
+/* bbl / bblforce in place: src/lbkernel.c:58-618 restated per solid node. */
+template <typename T, bool MOVING, bool FORCE>
+__global__ void __launch_bounds__(256)
+k_bbl(T *__restrict__ a, const int32_t *__restrict__ mask,
+      const double *__restrict__ ux, const double *__restrict__ uy,
+      const double *__restrict__ uz, double *__restrict__ fx,
+      double *__restrict__ fy, double *__restrict__ fz, Geom G) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= G.plane) return;
+  const int i = blockIdx.y, j = t / G.esk, k = t - j * G.esk;
+  const long x = (long)i * G.plane + t, N = G.N;
+  const int m = mask[x];
+  if (m <= 0) return;
+  const AxisOff oi = axis_off(i, G.esi, G.plane);
+  const AxisOff oj = axis_off(j, G.esj, G.esk);
+  const AxisOff ok = axis_off(k, G.esk, 1);
+  double Fx = 0.0, Fy = 0.0, Fz = 0.0;
+  Links<1, 15>::run([&](auto LL) {
+    constexpr int l = decltype(LL)::value;
+    if ((m >> (l - 1)) & 1) {
+      const long f = x + nb_off<l, +1>(oi, oj, ok);
+      const double av = (double)a[(long)l * N + x];
+      const double bv = (double)a[(long)OPP(l) * N + f];
+      double fac = 0.0;
+      if (MOVING)
+        fac = (double)WL(l) * ((CX(l) ? CX(l) * ux[x] : 0.0) + (CY(l) ? CY(l) * uy[x] : 0.0) +
+                               (CZ(l) ? CZ(l) * uz[x] : 0.0));
+      a[(long)l * N + x] = (T)(bv + fac);
+      a[(long)OPP(l) * N + f] = (T)(av - fac);
+      if (FORCE) {
+        const double g = 2.0 * (av - bv - fac);
+        if (CX(l)) Fx += CX(l) * g;
+        if (CY(l)) Fy += CY(l) * g;
+        if (CZ(l)) Fz += CZ(l) * g;
+      }
+    }
+  });
+  if (FORCE) { fx[x] += Fx; fy[x] += Fy; fz[x] += Fz; }
+}
+
+/* propagate: dst_l(x) = src_l(x - c_l), src/lbkernel.c:623-762. */
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_propagate(const T *__restrict__ src, T *__restrict__ dst, Geom G) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= G.plane) return;
+  const int i = blockIdx.y, j = t / G.esk, k = t - j * G.esk;
+  const long x = (long)i * G.plane + t, N = G.N;
+  const AxisOff oi = axis_off(i, G.esi, G.plane);
+  const AxisOff oj = axis_off(j, G.esj, G.esk);
+  const AxisOff ok = axis_off(k, G.esk, 1);
+  dst[x] = src[x];
+  Links<1, 15>::run([&](auto LL) {
+    constexpr int l = decltype(LL)::value;
+    dst[(long)l * N + x] = src[(long)l * N + x + nb_off<l, -1>(oi, oj, ok)];
+  });
+}
+
+/* collide in place, src/lbkernel.c:769-877. */
+template <typename T, int BF>
+__global__ void __launch_bounds__(256)
+k_collide(T *__restrict__ a, const uint16_t *__restrict__ flags,
+          const T *__restrict__ bx, const T *__restrict__ by, const T *__restrict__ bz,
+          Geom G, PhysArgs P) {
+  const long x = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= G.N) return;
+  const bool solid = (flags[x] & FLAG_SOLID) != 0;
+  if (P.skip_solid && solid) return;
+  const long N = G.N;
+  T v[15];
+#pragma unroll
+  for (int l = 0; l < 15; l++) v[l] = a[(long)l * N + x];
+  T jbx = T(0), jby = T(0), jbz = T(0);
+  if (BF != 0 && !solid) {
+    jbx = T(P.bfx); jby = T(P.bfy); jbz = T(P.bfz);
+    if (BF == 2) { jbx += bx[x]; jby += by[x]; jbz += bz[x]; }
+  }
+  collide_node<T>(v, jbx, jby, jbz, P);
+#pragma unroll
+  for (int l = 0; l < 15; l++) a[(long)l * N + x] = v[l];
+}
+
+/* ---------------------------------------------------------------------- */
+/* Geometry tables.                                                        */
+
+/* makemask, src/lbobjs.c:526-560: bit l-1 on a solid node iff the node at
+ * +c_l is fluid.  Bit-exact integer work. */
+__global__ void __launch_bounds__(256)
+k_makemask(const int32_t *__restrict__ objidx, int32_t *__restrict__ mask, Geom G) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= G.plane) return;
+  const int i = blockIdx.y, j = t / G.esk, k = t - j * G.esk;
+  const long x = (long)i * G.plane + t;
+  int m = 0;
+  if (objidx[x] >= 0) {
+    const AxisOff oi = axis_off(i, G.esi, G.plane);
+    const AxisOff oj = axis_off(j, G.esj, G.esk);
+    const AxisOff ok = axis_off(k, G.esk, 1);
+    Links<1, 15>::run([&](auto LL) {
+      constexpr int l = decltype(LL)::value;
+      if (objidx[x + nb_off<l, +1>(oi, oj, ok)] < 0) m |= 1 << (l - 1);
+    });
+  }
+  mask[x] = m;
+}
+
+/* Pull flags: bit l-1 = bounce along l at x, i.e. bit l-1 of mask(x - c_l)
+ * or bit lbar-1 of mask(x); bit 14 = solid. */
+__global__ void __launch_bounds__(256)
+k_flags(const int32_t *__restrict__ objidx, const int32_t *__restrict__ mask,
+        uint16_t *__restrict__ flags, Geom G) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= G.plane) return;
+  const int i = blockIdx.y, j = t / G.esk, k = t - j * G.esk;
+  const long x = (long)i * G.plane + t;
+  const AxisOff oi = axis_off(i, G.esi, G.plane);
+  const AxisOff oj = axis_off(j, G.esj, G.esk);
+  const AxisOff ok = axis_off(k, G.esk, 1);
+  const int mx = mask[x];
+  unsigned f = (objidx[x] >= 0) ? FLAG_SOLID : 0u;
+  Links<1, 15>::run([&](auto LL) {
+    constexpr int l = decltype(LL)::value;
+    const int ms = mask[x + nb_off<l, -1>(oi, oj, ok)];
+    if (((ms >> (l - 1)) & 1) || ((mx >> (OPP(l) - 1)) & 1)) f |= 1u << (l - 1);
+  });
+  flags[x] = (uint16_t)f;
+}
+
+/* Wall velocity of solid nodes from the object table, src/lbobjs.c:565-589:
+ * U = rho0 (v + w x r)/12, r = node - centre (no minimum image). */
+__device__ __forceinline__ void obj_velocity(const double *__restrict__ tab, int o,
+                                             int i, int j, int k, double rho0,
+                                             double &Ux, double &Uy, double &Uz) {
+  const double *p = tab + (long)o * SLBD_OBJ_STRIDE;
+  const double rx = i - p[0], ry = j - p[1], rz = k - p[2];
+  Ux = rho0 * (p[3] + p[7] * rz - p[8] * ry) / 12.0;
+  Uy = rho0 * (p[4] + p[8] * rx - p[6] * rz) / 12.0;
+  Uz = rho0 * (p[5] + p[6] * ry - p[7] * rx) / 12.0;
+}
+
+__global__ void __launch_bounds__(256)
+k_vspread(const int32_t *__restrict__ objidx, const double *__restrict__ tab,
+          double rho0, double *__restrict__ ux, double *__restrict__ uy,
+          double *__restrict__ uz, Geom G) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= G.plane) return;
+  const int i = blockIdx.y, j = t / G.esk, k = t - j * G.esk;
+  const long x = (long)i * G.plane + t;
+  const int o = objidx[x];
+  double U0 = 0.0, U1 = 0.0, U2 = 0.0;
+  if (o >= 0) obj_velocity(tab, o, i, j, k, rho0, U0, U1, U2);
+  ux[x] = U0; uy[x] = U1; uz[x] = U2;
+}
+
+/* ---------------------------------------------------------------------- */
+/* Reductions.                                                             */
+
+#define RED_BLOCKS 1024
+
+template <int NV>
+__device__ __forceinline__ void block_reduce_store(double (&s)[NV], double *__restrict__ part) {
+  __shared__ double red[NV][256 / 64];
+#pragma unroll
+  for (int q = 0; q < NV; q++) {
+    double v = s[q];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    s[q] = v;
+  }
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (lane == 0) {
+#pragma unroll
+    for (int q = 0; q < NV; q++) red[q][wv] = s[q];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int q = 0; q < NV; q++) {
+      double v = 0.0;
+      for (int w = 0; w < (int)(blockDim.x >> 6); w++) v += red[q][w];
+      part[(long)blockIdx.x * NV + q] = v;
+    }
+  }
+}
+
+/* Final pass: one block sums nparts rows of NV doubles in a fixed order. */
+template <int NV>
+__global__ void __launch_bounds__(256)
+k_final(const double *__restrict__ part, int nparts, double *__restrict__ out) {
+  double s[NV];
+#pragma unroll
+  for (int q = 0; q < NV; q++) s[q] = 0.0;
+  for (int b = threadIdx.x; b < nparts; b += blockDim.x)
+#pragma unroll
+    for (int q = 0; q < NV; q++) s[q] += part[(long)b * NV + q];
+  __shared__ double red[NV][256 / 64];
+#pragma unroll
+  for (int q = 0; q < NV; q++) {
+    double v = s[q];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    s[q] = v;
+  }
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (lane == 0)
+#pragma unroll
+    for (int q = 0; q < NV; q++) red[q][wv] = s[q];
+  __syncthreads();
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int q = 0; q < NV; q++) {
+      double v = 0.0;
+      for (int w = 0; w < (int)(blockDim.x >> 6); w++) v += red[q][w];
+      out[q] = v;
+    }
+}
+
+/* Node momentum and density for measurement (FP64 from T populations). */
+template <typename T>
+__device__ __forceinline__ void node_rho_j(const T *__restrict__ a, long x, long N,
+                                           double &d, double &jx, double &jy, double &jz) {
+  double v[15];
+#pragma unroll
+  for (int l = 0; l < 15; l++) v[l] = (double)a[(long)l * N + x];
+  d = 0.0; jx = 0.0; jy = 0.0; jz = 0.0;
+#pragma unroll
+  for (int l = 0; l < 15; l++) {
+    d += v[l];
+    if (CX(l)) jx += CX(l) * v[l];
+    if (CY(l)) jy += CY(l) * v[l];
+    if (CZ(l)) jz += CZ(l) * v[l];
+  }
+}
+
+/* accvmean sums, src/lbmeasure.c:58-106. */
+template <typename T, int BF>
+__global__ void __launch_bounds__(256)
+k_vmean(const T *__restrict__ a, const uint16_t *__restrict__ flags,
+        const T *__restrict__ bx, const T *__restrict__ by, const T *__restrict__ bz,
+        const double *__restrict__ psi, double fac, Geom G, PhysArgs P,
+        double *__restrict__ part) {
+  double s[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  const long N = G.N;
+  for (long x = (long)blockIdx.x * blockDim.x + threadIdx.x; x < N;
+       x += (long)gridDim.x * blockDim.x) {
+    if (flags[x] & FLAG_SOLID) continue;
+    double d, jx, jy, jz;
+    node_rho_j<T>(a, x, N, d, jx, jy, jz);
+    double b0 = 0.0, b1 = 0.0, b2 = 0.0;
+    if (BF != 0) {
+      b0 = P.bfx; b1 = P.bfy; b2 = P.bfz;
+      if (BF == 2) { b0 += (double)bx[x]; b1 += (double)by[x]; b2 += (double)bz[x]; }
+    }
+    const double rho = P.rho0 + d;
+    double w = 1.0;
+    if (psi) { w = psi[x]; w *= w; }
+    s[0] += w * ((jx + fac * b0) / rho);
+    s[1] += w * ((jy + fac * b1) / rho);
+    s[2] += w * ((jz + fac * b2) / rho);
+    s[3] += w;
+    s[4] += 1.0;
+  }
+  block_reduce_store<5>(s, part);
+}
+
+/* vgrab, src/lbmeasure.c:128-166; vzero :110-121. */
+template <typename T, int BF>
+__global__ void __launch_bounds__(256)
+k_vgrab(const T *__restrict__ a, const uint16_t *__restrict__ flags,
+        const T *__restrict__ bx, const T *__restrict__ by, const T *__restrict__ bz,
+        double fac, double prefac, Geom G, PhysArgs P, double *__restrict__ ux,
+        double *__restrict__ uy, double *__restrict__ uz, int zero) {
+  const long x = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= G.N) return;
+  if (flags[x] & FLAG_SOLID) return;
+  if (zero) { ux[x] = 0.0; uy[x] = 0.0; uz[x] = 0.0; return; }
+  double d, jx, jy, jz;
+  node_rho_j<T>(a, x, G.N, d, jx, jy, jz);
+  double b0 = 0.0, b1 = 0.0, b2 = 0.0;
+  if (BF != 0) {
+    b0 = P.bfx; b1 = P.bfy; b2 = P.bfz;
+    if (BF == 2) { b0 += (double)bx[x]; b1 += (double)by[x]; b2 += (double)bz[x]; }
+  }
+  const double rho = P.rho0 + d;
+  ux[x] += prefac * (jx + fac * b0) / rho;
+  uy[x] += prefac * (jy + fac * b1) / rho;
+  uz[x] += prefac * (jz + fac * b2) / rho;
+}
+
+/* Velocity residual: sums of |u - u_prev|^2 and |u|^2 over fluid nodes,
+ * u = (j - b/2)/rho post-collide; u_prev <- u. */
+template <typename T, int BF>
+__global__ void __launch_bounds__(256)
+k_residual(const T *__restrict__ a, const uint16_t *__restrict__ flags,
+           const T *__restrict__ bx, const T *__restrict__ by, const T *__restrict__ bz,
+           Geom G, PhysArgs P, float *__restrict__ up, double *__restrict__ part) {
+  double s[2] = {0.0, 0.0};
+  const long N = G.N;
+  for (long x = (long)blockIdx.x * blockDim.x + threadIdx.x; x < N;
+       x += (long)gridDim.x * blockDim.x) {
+    if (flags[x] & FLAG_SOLID) continue;
+    double d, jx, jy, jz;
+    node_rho_j<T>(a, x, N, d, jx, jy, jz);
+    double b0 = 0.0, b1 = 0.0, b2 = 0.0;
+    if (BF != 0) {
+      b0 = P.bfx; b1 = P.bfy; b2 = P.bfz;
+      if (BF == 2) { b0 += (double)bx[x]; b1 += (double)by[x]; b2 += (double)bz[x]; }
+    }
+    const double rho = P.rho0 + d;
+    const double u0 = (jx - 0.5 * b0) / rho, u1 = (jy - 0.5 * b1) / rho, u2 = (jz - 0.5 * b2) / rho;
+    const double e0 = u0 - up[x], e1 = u1 - up[N + x], e2 = u2 - up[2 * N + x];
+    s[0] += e0 * e0 + e1 * e1 + e2 * e2;
+    s[1] += u0 * u0 + u1 * u1 + u2 * u2;
+    up[x] = (float)u0; up[N + x] = (float)u1; up[2 * N + x] = (float)u2;
+  }
+  block_reduce_store<2>(s, part);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_sum_pops(const T *__restrict__ a, const uint16_t *__restrict__ flags, Geom G,
+           double *__restrict__ part) {
+  double s[15];
+#pragma unroll
+  for (int l = 0; l < 15; l++) s[l] = 0.0;
+  const long N = G.N;
+  for (long x = (long)blockIdx.x * blockDim.x + threadIdx.x; x < N;
+       x += (long)gridDim.x * blockDim.x) {
+    if (flags[x] & FLAG_SOLID) continue;
+#pragma unroll
+    for (int l = 0; l < 15; l++) s[l] += (double)a[(long)l * N + x];
+  }
+  block_reduce_store<15>(s, part);
+}
+
+/* distribf, src/lbmeasure.c:236-263: per object F = sum f, T = sum r x f.
+ * Small object counts accumulate in LDS first. */
+#define DF_LDS_OBJ 96
+__global__ void __launch_bounds__(256)
+k_distribf(const int32_t *__restrict__ objidx, const double *__restrict__ fx,
+           const double *__restrict__ fy, const double *__restrict__ fz,
+           const double *__restrict__ tab, int nobj, Geom G,
+           double *__restrict__ acc /* 6*nobj */) {
+  __shared__ double sacc[DF_LDS_OBJ * 6];
+  const bool use_lds = nobj <= DF_LDS_OBJ;
+  if (use_lds) {
+    for (int q = threadIdx.x; q < nobj * 6; q += blockDim.x) sacc[q] = 0.0;
+    __syncthreads();
+  }
+  const long N = G.N;
+  for (long x = (long)blockIdx.x * blockDim.x + threadIdx.x; x < N;
+       x += (long)gridDim.x * blockDim.x) {
+    const int o = objidx[x];
+    if (o < 0) continue;
+    const double f0 = fx[x], f1 = fy[x], f2 = fz[x];
+    if (f0 == 0.0 && f1 == 0.0 && f2 == 0.0) continue;
+    const int i = (int)(x / G.plane);
+    const int rem = (int)(x - (long)i * G.plane);
+    const int j = rem / G.esk, k = rem - j * G.esk;
+    const double *p = tab + (long)o * SLBD_OBJ_STRIDE;
+    const double rx = i - p[0], ry = j - p[1], rz = k - p[2];
+    const double t0 = ry * f2 - rz * f1, t1 = rz * f0 - rx * f2, t2 = rx * f1 - ry * f0;
+    double *dstp = use_lds ? sacc + 6 * o : acc + 6 * (long)o;
+    atomicAdd(dstp + 0, f0); atomicAdd(dstp + 1, f1); atomicAdd(dstp + 2, f2);
+    atomicAdd(dstp + 3, t0); atomicAdd(dstp + 4, t1); atomicAdd(dstp + 5, t2);
+  }
+  if (use_lds) {
+    __syncthreads();
+    for (int q = threadIdx.x; q < nobj * 6; q += blockDim.x)
+      if (sacc[q] != 0.0) atomicAdd(acc + q, sacc[q]);
+  }
+}
+
+/* ---------------------------------------------------------------------- */
+/* Moment propagation (src/mpkernel.c), gather form.                      */
+
+/* mprhoinit, src/mpmisc.c:64-79 */
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_mprho(const T *__restrict__ a, const uint16_t *__restrict__ flags, double rho0,
+        T *__restrict__ rho, Geom G) {
+  const long x = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= G.N) return;
+  double r = rho0;
+  if (!(flags[x] & FLAG_SOLID)) {
+#pragma unroll
+    for (int l = 0; l < 15; l++) r += (double)a[(long)l * G.N + x];
+  }
+  rho[x] = (T)r;
+}
+
+/* MP flags: bit l (0..14) = node x - c_l is fluid; bit 16+l (1..14) = node
+ * s = x - c_l is a solid with boundary link l that reflects (mask bit l-1
+ * set, adsorption bit 0x4000 clear). */
+__global__ void __launch_bounds__(256)
+k_mpflags(const int32_t *__restrict__ objidx, const int32_t *__restrict__ mask,
+          uint32_t *__restrict__ mpf, Geom G) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= G.plane) return;
+  const int i = blockIdx.y, j = t / G.esk, k = t - j * G.esk;
+  const long x = (long)i * G.plane + t;
+  const AxisOff oi = axis_off(i, G.esi, G.plane);
+  const AxisOff oj = axis_off(j, G.esj, G.esk);
+  const AxisOff ok = axis_off(k, G.esk, 1);
+  uint32_t f = (objidx[x] < 0) ? 1u : 0u;
+  Links<1, 15>::run([&](auto LL) {
+    constexpr int l = decltype(LL)::value;
+    const long s = x + nb_off<l, -1>(oi, oj, ok);
+    if (objidx[s] < 0) {
+      f |= 1u << l;
+    } else {
+      const int m = mask[s];
+      if (m && !(m & 0x4000) && ((m >> (l - 1)) & 1)) f |= 1u << (16 + l);
+    }
+  });
+  mpf[x] = f;
+}
+
+struct MPArgs {
+  double rho0, alpha, pads;
+  int axis;
+};
+
+/* One moment-propagation step for every node x:
+ *   new(x) = sum_l [x - c_l fluid] pi_l(x - c_l) P(x - c_l)
+ *          + sum_l [x - c_l reflecting solid] (1 - pads)(pi_lbar(x)
+ *                 + w_l c_l.U(x - c_l)/rho(x)) P(x)
+ *   pi_l(y) = a_l(y)/rho(y) + (w_l/72)(rho0/rho(y) - alpha) (+ alpha, l = 0)
+ * then the update rule of src/mpkernel.c:190-197 (fluid replace, adsorbing
+ * site accumulate, other solid clear; partial adsorption keeps pads*new). */
+template <typename T, int NMOM, bool MOVING>
+__global__ void __launch_bounds__(256)
+k_mpstep(const T *__restrict__ a, const T *__restrict__ rho,
+         const T *__restrict__ pin, T *__restrict__ pout,
+         const T *__restrict__ m1in, T *__restrict__ m1out,
+         const T *__restrict__ m2in, T *__restrict__ m2out,
+         const uint32_t *__restrict__ mpf, const int32_t *__restrict__ mask,
+         const int32_t *__restrict__ objidx, const double *__restrict__ tab,
+         Geom G, MPArgs A) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= G.plane) return;
+  const int i = blockIdx.y, j = t / G.esk, k = t - j * G.esk;
+  const long x = (long)i * G.plane + t, N = G.N;
+  const AxisOff oi = axis_off(i, G.esi, G.plane);
+  const AxisOff oj = axis_off(j, G.esj, G.esk);
+  const AxisOff ok = axis_off(k, G.esk, 1);
+  const uint32_t f = mpf[x];
+  const T r0 = T(A.rho0), al = T(A.alpha);
+  T np = T(0), n1 = T(0), n2 = T(0);
+  /* rest link: x itself */
+  if (f & 1u) {
+    const T r = rho[x], P = pin[x];
+    const T pr = a[x] / r + T(16.0 / 72.0) * (r0 / r - al) + al;
+    np += pr * P;
+    if (NMOM > 1) { n1 += pr * m1in[x]; n2 += pr * m2in[x]; }
+  }
+  Links<1, 15>::run([&](auto LL) {
+    constexpr int l = decltype(LL)::value;
+    if (f & (1u << l)) {
+      const long y = x + nb_off<l, -1>(oi, oj, ok);
+      const T r = rho[y], P = pin[y];
+      const T pr = a[(long)l * N + y] / r + T(WL(l) / 72.0) * (r0 / r - al);
+      np += pr * P;
+      if (NMOM > 1) {
+        const int dd = (A.axis == 0) ? CX(l) : (A.axis == 1) ? CY(l) : CZ(l);
+        const T q1 = m1in[y];
+        n1 += pr * (q1 + T(dd) * P);
+        n2 += pr * (m2in[y] + T(2 * dd) * q1 + T(dd * dd) * P);
+      }
+    }
+  });
+  if (f >> 17) {
+    const T r = rho[x], P = pin[x];
+    const T keep = T(1.0 - A.pads);
+    Links<1, 15>::run([&](auto LL) {
+      constexpr int l = decltype(LL)::value;
+      if (f & (1u << (16 + l))) {
+        T pr = a[(long)OPP(l) * N + x] / r + T(WL(l) / 72.0) * (r0 / r - al);
+        if (MOVING) {
+          const long s = x + nb_off<l, -1>(oi, oj, ok);
+          const int si = (i - CX(l) + G.esi) % G.esi;
+          const int sj = (j - CY(l) + G.esj) % G.esj;
+          const int sk = (k - CZ(l) + G.esk) % G.esk;
+          double Ux, Uy, Uz;
+          obj_velocity(tab, objidx[s], si, sj, sk, A.rho0, Ux, Uy, Uz);
+          const double cu = (double)WL(l) * (CX(l) * Ux + CY(l) * Uy + CZ(l) * Uz);
+          pr += (T)(cu / (double)r);
+        }
+        pr *= keep;
+        np += pr * P;
+        if (NMOM > 1) { n1 += pr * m1in[x]; n2 += pr * m2in[x]; }
+      }
+    });
+  }
+  T o0, o1 = T(0), o2 = T(0);
+  if (f & 1u) {
+    o0 = np;
+    if (NMOM > 1) { o1 = n1; o2 = n2; }
+  } else {
+    const int m = mask[x];
+    T frac = T(0);
+    if (m & 0x4000) frac = T(1);
+    else if (m != 0 && A.pads > 0.0) frac = T(A.pads);
+    if (frac != T(0)) {
+      o0 = pin[x] + frac * np;
+      if (NMOM > 1) { o1 = m1in[x] + frac * n1; o2 = m2in[x] + frac * n2; }
+    } else {
+      o0 = T(0);
+    }
+  }
+  pout[x] = o0;
+  if (NMOM > 1) { m1out[x] = o1; m2out[x] = o2; }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_mpsums(const T *__restrict__ p, const T *__restrict__ m1, const T *__restrict__ m2,
+         const uint16_t *__restrict__ flags, const int32_t *__restrict__ mask,
+         int nmom, Geom G, double *__restrict__ part) {
+  double s[6] = {0, 0, 0, 0, 0, 0};
+  const long N = G.N;
+  for (long x = (long)blockIdx.x * blockDim.x + threadIdx.x; x < N;
+       x += (long)gridDim.x * blockDim.x) {
+    const double v = (double)p[x];
+    if (flags[x] & FLAG_SOLID) {
+      s[1] += v;
+    } else {
+      s[0] += v;
+      s[5] += 1.0;
+      if (nmom > 1) { s[3] += (double)m1[x]; s[4] += (double)m2[x]; }
+    }
+    if (mask[x] & 0x4000) s[2] += v;
+  }
+  block_reduce_store<6>(s, part);
+}
+
+/* ---------------------------------------------------------------------- */
+/* Elementwise helpers.                                                    */
+
+template <typename S, typename D>
+__global__ void k_convert(const S *__restrict__ s, D *__restrict__ d, long n) {
+  const long x = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (x < n) d[x] = (D)s[x];
+}
+template <typename T>
+__global__ void k_scale(T *__restrict__ d, long n, double s) {
+  const long x = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (x < n) d[x] = (T)((double)d[x] * s);
+}
+
+/* ---------------------------------------------------------------------- */
+/* Host side.                                                              */
+
+struct slbd_lattice {
+  int esi, esj, esk, prec;
+  long N;
+  Geom G;
+  hipStream_t stream;
+  void *pops[2];
+  int cur;
+  uint16_t *flags;
+  int32_t *objidx, *mask;
+  double *ux, *uy, *uz;
+  double *fx, *fy, *fz;
+  void *bx, *by, *bz;
+  double *part, *dout;      /* reduction partials and result */
+  double *psi;              /* device copy of psi when used */
+  double *tab;              /* object table */
+  int tab_cap;
+  double *acc;              /* distribf accumulators */
+  int acc_cap;
+  float *uprev;             /* residual previous velocity */
+  int uprev_valid;
+  /* moment propagation: frozen density and link flags (per lattice) */
+  void *rho;
+  uint32_t *mpf;
+  hipEvent_t ev0, ev1;
+};
+
+struct slbd_mp {
+  void *f[3][2];   /* P, m1, m2 ping-pong pairs */
+  int cur, nmom, prec;
+  long N;
+};
+
+static size_t tsize(int prec) { return prec == 32 ? sizeof(float) : sizeof(double); }
+
+static void *dalloc(size_t bytes) {
+  void *p = NULL;
+  if (bytes == 0) bytes = 8;
+  HIPCHECK(hipMalloc(&p, bytes));
+  return p;
+}
+
+static dim3 plane_grid(const slbd_lattice *L) {
+  return dim3((unsigned)((L->G.plane + 255) / 256), (unsigned)L->esi, 1);
+}
+static dim3 lin_grid(long n) { return dim3((unsigned)((n + 255) / 256), 1, 1); }
+
+extern "C" int slbd_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+extern "C" const char *slbd_arch(void) {
+  static char name[128];
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, 0) != hipSuccess) return "none";
+  snprintf(name, sizeof name, "%s", prop.gcnArchName);
+  return name;
+}
+
+extern "C" slbd_lattice *slbd_create(int esi, int esj, int esk, int prec) {
+  if (slbd_device_count() < 1) {
+    fprintf(stderr, "sunlightlb: no HIP device is visible; the GPU library cannot run\n");
+    exit(EXIT_FAILURE);
+  }
+  if (prec != 32 && prec != 64) prec = 64;
+  slbd_lattice *L = (slbd_lattice *)calloc(1, sizeof(slbd_lattice));
+  L->esi = esi; L->esj = esj; L->esk = esk; L->prec = prec;
+  L->N = (long)esi * esj * esk;
+  L->G.esi = esi; L->G.esj = esj; L->G.esk = esk; L->G.plane = esj * esk; L->G.N = L->N;
+  HIPCHECK(hipStreamCreateWithFlags(&L->stream, hipStreamNonBlocking));
+  HIPCHECK(hipEventCreate(&L->ev0));
+  HIPCHECK(hipEventCreate(&L->ev1));
+  const size_t pb = 15 * (size_t)L->N * tsize(prec);
+  L->pops[0] = dalloc(pb);
+  L->pops[1] = dalloc(pb);
+  HIPCHECK(hipMemsetAsync(L->pops[0], 0, pb, L->stream));
+  HIPCHECK(hipMemsetAsync(L->pops[1], 0, pb, L->stream));
+  L->flags = (uint16_t *)dalloc(sizeof(uint16_t) * L->N);
+  L->objidx = (int32_t *)dalloc(sizeof(int32_t) * L->N);
+  L->mask = (int32_t *)dalloc(sizeof(int32_t) * L->N);
+  HIPCHECK(hipMemsetAsync(L->flags, 0, sizeof(uint16_t) * L->N, L->stream));
+  HIPCHECK(hipMemsetAsync(L->objidx, 0xff, sizeof(int32_t) * L->N, L->stream));
+  HIPCHECK(hipMemsetAsync(L->mask, 0, sizeof(int32_t) * L->N, L->stream));
+  L->ux = (double *)dalloc(sizeof(double) * L->N);
+  L->uy = (double *)dalloc(sizeof(double) * L->N);
+  L->uz = (double *)dalloc(sizeof(double) * L->N);
+  HIPCHECK(hipMemsetAsync(L->ux, 0, sizeof(double) * L->N, L->stream));
+  HIPCHECK(hipMemsetAsync(L->uy, 0, sizeof(double) * L->N, L->stream));
+  HIPCHECK(hipMemsetAsync(L->uz, 0, sizeof(double) * L->N, L->stream));
+  L->part = (double *)dalloc(sizeof(double) * RED_BLOCKS * 16);
+  L->dout = (double *)dalloc(sizeof(double) * 16);
+  HIPCHECK(hipStreamSynchronize(L->stream));
+  return L;
+}
+
+static void free_mp(slbd_lattice *L) {
+  if (L->rho) { HIPCHECK(hipFree(L->rho)); L->rho = NULL; }
+  if (L->mpf) { HIPCHECK(hipFree(L->mpf)); L->mpf = NULL; }
+}
+
+extern "C" void slbd_destroy(slbd_lattice *L) {
+  if (!L) return;
+  HIPCHECK(hipStreamSynchronize(L->stream));
+  void *ps[] = {L->pops[0], L->pops[1], L->flags, L->objidx, L->mask, L->ux, L->uy, L->uz,
+                L->fx, L->fy, L->fz, L->bx, L->by, L->bz, L->part, L->dout, L->psi,
+                L->tab, L->acc, L->uprev};
+  for (size_t q = 0; q < sizeof ps / sizeof ps[0]; q++)
+    if (ps[q]) HIPCHECK(hipFree(ps[q]));
+  free_mp(L);
+  HIPCHECK(hipEventDestroy(L->ev0));
+  HIPCHECK(hipEventDestroy(L->ev1));
+  HIPCHECK(hipStreamDestroy(L->stream));
+  free(L);
+}
+
+extern "C" int slbd_precision(const slbd_lattice *L) { return L->prec; }
+extern "C" void slbd_sync(slbd_lattice *L) { HIPCHECK(hipStreamSynchronize(L->stream)); }
+
+extern "C" size_t slbd_bytes(const slbd_lattice *L) {
+  size_t b = 2 * 15 * (size_t)L->N * tsize(L->prec);
+  b += (size_t)L->N * (2 + 4 + 4 + 24);
+  if (L->fx) b += (size_t)L->N * 24;
+  if (L->bx) b += (size_t)L->N * 3 * tsize(L->prec);
+  return b;
+}
+
+/* Populations: host AoS double <-> device SoA T, transposed on the host one
+ * link at a time. */
+extern "C" void slbd_put_pops(slbd_lattice *L, const double *aos) {
+  const long N = L->N;
+  const size_t ts = tsize(L->prec);
+  void *tmp = malloc(N * ts);
+  char *dev = (char *)L->pops[L->cur];
+  for (int l = 0; l < 15; l++) {
+    if (L->prec == 32) {
+      float *t = (float *)tmp;
+      for (long n = 0; n < N; n++) t[n] = (float)aos[15 * n + l];
+    } else {
+      double *t = (double *)tmp;
+      for (long n = 0; n < N; n++) t[n] = aos[15 * n + l];
+    }
+    HIPCHECK(hipMemcpyAsync(dev + (size_t)l * N * ts, tmp, N * ts, hipMemcpyHostToDevice, L->stream));
+    HIPCHECK(hipStreamSynchronize(L->stream));
+  }
+  free(tmp);
+}
+
+extern "C" void slbd_get_pops(slbd_lattice *L, double *aos) {
+  const long N = L->N;
+  const size_t ts = tsize(L->prec);
+  void *tmp = malloc(N * ts);
+  const char *dev = (const char *)L->pops[L->cur];
+  for (int l = 0; l < 15; l++) {
+    HIPCHECK(hipMemcpyAsync(tmp, dev + (size_t)l * N * ts, N * ts, hipMemcpyDeviceToHost, L->stream));
+    HIPCHECK(hipStreamSynchronize(L->stream));
+    if (L->prec == 32) {
+      const float *t = (const float *)tmp;
+      for (long n = 0; n < N; n++) aos[15 * n + l] = (double)t[n];
+    } else {
+      const double *t = (const double *)tmp;
+      for (long n = 0; n < N; n++) aos[15 * n + l] = t[n];
+    }
+  }
+  free(tmp);
+}
+
+extern "C" void slbd_zero_pops(slbd_lattice *L) {
+  HIPCHECK(hipMemsetAsync(L->pops[L->cur], 0, 15 * (size_t)L->N * tsize(L->prec), L->stream));
+  HIPCHECK(hipStreamSynchronize(L->stream));
+}
+
+extern "C" int slbd_set_precision(slbd_lattice *L, int prec, double *scratch_aos) {
+  if (prec != 32 && prec != 64) return -1;
+  if (prec == L->prec) return 0;
+  slbd_get_pops(L, scratch_aos);
+  HIPCHECK(hipStreamSynchronize(L->stream));
+  HIPCHECK(hipFree(L->pops[0]));
+  HIPCHECK(hipFree(L->pops[1]));
+  if (L->bx) { HIPCHECK(hipFree(L->bx)); HIPCHECK(hipFree(L->by)); HIPCHECK(hipFree(L->bz)); }
+  L->bx = L->by = L->bz = NULL;
+  free_mp(L);
+  L->prec = prec;
+  const size_t pb = 15 * (size_t)L->N * tsize(prec);
+  L->pops[0] = dalloc(pb);
+  L->pops[1] = dalloc(pb);
+  L->cur = 0;
+  slbd_put_pops(L, scratch_aos);
+  return 1;
+}
+
+static void copy_in(void *dev, const void *host, size_t bytes, hipStream_t s) {
+  HIPCHECK(hipMemcpyAsync(dev, host, bytes, hipMemcpyHostToDevice, s));
+  HIPCHECK(hipStreamSynchronize(s));
+}
+static void copy_out(void *host, const void *dev, size_t bytes, hipStream_t s) {
+  HIPCHECK(hipMemcpyAsync(host, dev, bytes, hipMemcpyDeviceToHost, s));
+  HIPCHECK(hipStreamSynchronize(s));
+}
+
+extern "C" void slbd_put_objidx(slbd_lattice *L, const int32_t *objidx) {
+  copy_in(L->objidx, objidx, sizeof(int32_t) * L->N, L->stream);
+}
+extern "C" void slbd_get_objidx(slbd_lattice *L, int32_t *objidx) {
+  copy_out(objidx, L->objidx, sizeof(int32_t) * L->N, L->stream);
+}
+extern "C" void slbd_makemask(slbd_lattice *L) {
+  hipLaunchKernelGGL(k_makemask, plane_grid(L), dim3(256), 0, L->stream, L->objidx, L->mask, L->G);
+  LAUNCHCHECK();
+}
+extern "C" void slbd_put_mask(slbd_lattice *L, const int32_t *mask) {
+  copy_in(L->mask, mask, sizeof(int32_t) * L->N, L->stream);
+}
+extern "C" void slbd_get_mask(slbd_lattice *L, int32_t *mask) {
+  copy_out(mask, L->mask, sizeof(int32_t) * L->N, L->stream);
+}
+extern "C" void slbd_build_flags(slbd_lattice *L) {
+  hipLaunchKernelGGL(k_flags, plane_grid(L), dim3(256), 0, L->stream, L->objidx, L->mask,
+                     L->flags, L->G);
+  LAUNCHCHECK();
+  if (L->mpf) {
+    hipLaunchKernelGGL(k_mpflags, plane_grid(L), dim3(256), 0, L->stream, L->objidx, L->mask,
+                       L->mpf, L->G);
+    LAUNCHCHECK();
+  }
+  HIPCHECK(hipStreamSynchronize(L->stream));
+}
+
+extern "C" void slbd_put_u(slbd_lattice *L, const double *ux, const double *uy, const double *uz) {
+  copy_in(L->ux, ux, sizeof(double) * L->N, L->stream);
+  copy_in(L->uy, uy, sizeof(double) * L->N, L->stream);
+  copy_in(L->uz, uz, sizeof(double) * L->N, L->stream);
+}
+extern "C" void slbd_get_u(slbd_lattice *L, double *ux, double *uy, double *uz) {
+  copy_out(ux, L->ux, sizeof(double) * L->N, L->stream);
+  copy_out(uy, L->uy, sizeof(double) * L->N, L->stream);
+  copy_out(uz, L->uz, sizeof(double) * L->N, L->stream);
+}
+
+extern "C" void slbd_put_bf(slbd_lattice *L, const double *bx, const double *by, const double *bz) {
+  const size_t ts = tsize(L->prec);
+  if (!L->bx) {
+    L->bx = dalloc(L->N * ts); L->by = dalloc(L->N * ts); L->bz = dalloc(L->N * ts);
+  }
+  const double *src[3] = {bx, by, bz};
+  void *dst[3] = {L->bx, L->by, L->bz};
+  double *stage = (double *)dalloc(sizeof(double) * L->N);
+  for (int q = 0; q < 3; q++) {
+    copy_in(stage, src[q], sizeof(double) * L->N, L->stream);
+    if (L->prec == 32)
+      hipLaunchKernelGGL((k_convert<double, float>), lin_grid(L->N), dim3(256), 0, L->stream,
+                         stage, (float *)dst[q], L->N);
+    else
+      hipLaunchKernelGGL((k_convert<double, double>), lin_grid(L->N), dim3(256), 0, L->stream,
+                         stage, (double *)dst[q], L->N);
+    LAUNCHCHECK();
+  }
+  HIPCHECK(hipStreamSynchronize(L->stream));
+  HIPCHECK(hipFree(stage));
+}
+
+static void ensure_f(slbd_lattice *L) {
+  if (L->fx) return;
+  L->fx = (double *)dalloc(sizeof(double) * L->N);
+  L->fy = (double *)dalloc(sizeof(double) * L->N);
+  L->fz = (double *)dalloc(sizeof(double) * L->N);
+  HIPCHECK(hipMemsetAsync(L->fx, 0, sizeof(double) * L->N, L->stream));
+  HIPCHECK(hipMemsetAsync(L->fy, 0, sizeof(double) * L->N, L->stream));
+  HIPCHECK(hipMemsetAsync(L->fz, 0, sizeof(double) * L->N, L->stream));
+}
+extern "C" void slbd_get_f(slbd_lattice *L, double *fx, double *fy, double *fz) {
+  if (!L->fx) {
+    memset(fx, 0, sizeof(double) * L->N); memset(fy, 0, sizeof(double) * L->N);
+    memset(fz, 0, sizeof(double) * L->N);
+    return;
+  }
+  copy_out(fx, L->fx, sizeof(double) * L->N, L->stream);
+  copy_out(fy, L->fy, sizeof(double) * L->N, L->stream);
+  copy_out(fz, L->fz, sizeof(double) * L->N, L->stream);
+}
+extern "C" void slbd_put_f(slbd_lattice *L, const double *fx, const double *fy, const double *fz) {
+  ensure_f(L);
+  copy_in(L->fx, fx, sizeof(double) * L->N, L->stream);
+  copy_in(L->fy, fy, sizeof(double) * L->N, L->stream);
+  copy_in(L->fz, fz, sizeof(double) * L->N, L->stream);
+}
+extern "C" void slbd_zero_f(slbd_lattice *L) {
+  ensure_f(L);
+  HIPCHECK(hipMemsetAsync(L->fx, 0, sizeof(double) * L->N, L->stream));
+  HIPCHECK(hipMemsetAsync(L->fy, 0, sizeof(double) * L->N, L->stream));
+  HIPCHECK(hipMemsetAsync(L->fz, 0, sizeof(double) * L->N, L->stream));
+}
+extern "C" void slbd_scale_f(slbd_lattice *L, double s) {
+  ensure_f(L);
+  double *fs[3] = {L->fx, L->fy, L->fz};
+  for (int q = 0; q < 3; q++) {
+    hipLaunchKernelGGL(k_scale<double>, lin_grid(L->N), dim3(256), 0, L->stream, fs[q], L->N, s);
+    LAUNCHCHECK();
+  }
+}
+extern "C" void slbd_scale_pops(slbd_lattice *L, double s) {
+  const long n = 15 * L->N;
+  if (L->prec == 32)
+    hipLaunchKernelGGL(k_scale<float>, lin_grid(n), dim3(256), 0, L->stream,
+                       (float *)L->pops[L->cur], n, s);
+  else
+    hipLaunchKernelGGL(k_scale<double>, lin_grid(n), dim3(256), 0, L->stream,
+                       (double *)L->pops[L->cur], n, s);
+  LAUNCHCHECK();
+}
+
+static PhysArgs phys(const slbd_params *P) {
+  PhysArgs a;
+  a.omega = 1.0 / P->tau;
+  a.comega = 1.0 - a.omega;
+  a.rho0 = P->rho0;
+  a.stokes = P->stokes_flow;
+  a.skip_solid = P->skip_solid;
+  a.bfx = P->bf[0]; a.bfy = P->bf[1]; a.bfz = P->bf[2];
+  return a;
+}
+
+static int bf_mode(const slbd_lattice *L, const slbd_params *P) {
+  if (P->bf_mode == 2 && L->bx) return 2;
+  if (P->bf_mode >= 1) return 1;
+  return 0;
+}
+
+template <typename T, bool MOVING, bool FORCE, int BF>
+static void launch_step_t(slbd_lattice *L, const PhysArgs &A) {
+  const T *src = (const T *)L->pops[L->cur];
+  T *dst = (T *)L->pops[L->cur ^ 1];
+  hipLaunchKernelGGL((k_step<T, MOVING, FORCE, BF>), plane_grid(L), dim3(256), 0, L->stream,
+                     src, dst, L->flags, L->ux, L->uy, L->uz, (const T *)L->bx,
+                     (const T *)L->by, (const T *)L->bz, L->fx, L->fy, L->fz, L->G, A);
+  LAUNCHCHECK();
+  L->cur ^= 1;
+}
+
+template <typename T>
+static void launch_step(slbd_lattice *L, const PhysArgs &A, bool moving, bool force, int bf) {
+#define SLB_STEP_CASE(M, F, B)                                      \
+  if (moving == M && force == F && bf == B) { launch_step_t<T, M, F, B>(L, A); return; }
+  SLB_STEP_CASE(false, false, 0) SLB_STEP_CASE(false, false, 1) SLB_STEP_CASE(false, false, 2)
+  SLB_STEP_CASE(false, true, 0) SLB_STEP_CASE(false, true, 1) SLB_STEP_CASE(false, true, 2)
+  SLB_STEP_CASE(true, false, 0) SLB_STEP_CASE(true, false, 1) SLB_STEP_CASE(true, false, 2)
+  SLB_STEP_CASE(true, true, 0) SLB_STEP_CASE(true, true, 1) SLB_STEP_CASE(true, true, 2)
+#undef SLB_STEP_CASE
+}
+
+extern "C" void slbd_step(slbd_lattice *L, const slbd_params *P, int nsteps, int force) {
+  const PhysArgs A = phys(P);
+  const int bf = bf_mode(L, P);
+  if (force) ensure_f(L);
+  for (int s = 0; s < nsteps; s++) {
+    if (L->prec == 32) launch_step<float>(L, A, P->moving != 0, force != 0, bf);
+    else launch_step<double>(L, A, P->moving != 0, force != 0, bf);
+  }
+}
+
+extern "C" double slbd_time_steps(slbd_lattice *L, const slbd_params *P, int nsteps) {
+  HIPCHECK(hipStreamSynchronize(L->stream));
+  HIPCHECK(hipEventRecord(L->ev0, L->stream));
+  slbd_step(L, P, nsteps, 0);
+  HIPCHECK(hipEventRecord(L->ev1, L->stream));
+  HIPCHECK(hipEventSynchronize(L->ev1));
+  float ms = 0.0f;
+  HIPCHECK(hipEventElapsedTime(&ms, L->ev0, L->ev1));
+  return (double)ms;
+}
+
+template <typename T>
+static void launch_bbl(slbd_lattice *L, bool moving, bool force) {
+  T *a = (T *)L->pops[L->cur];
+  if (moving && force)
+    hipLaunchKernelGGL((k_bbl<T, true, true>), plane_grid(L), dim3(256), 0, L->stream, a, L->mask,
+                       L->ux, L->uy, L->uz, L->fx, L->fy, L->fz, L->G);
+  else if (moving)
+    hipLaunchKernelGGL((k_bbl<T, true, false>), plane_grid(L), dim3(256), 0, L->stream, a, L->mask,
+                       L->ux, L->uy, L->uz, L->fx, L->fy, L->fz, L->G);
+  else if (force)
+    hipLaunchKernelGGL((k_bbl<T, false, true>), plane_grid(L), dim3(256), 0, L->stream, a, L->mask,
+                       L->ux, L->uy, L->uz, L->fx, L->fy, L->fz, L->G);
+  else
+    hipLaunchKernelGGL((k_bbl<T, false, false>), plane_grid(L), dim3(256), 0, L->stream, a, L->mask,
+                       L->ux, L->uy, L->uz, L->fx, L->fy, L->fz, L->G);
+  LAUNCHCHECK();
+}
+
+extern "C" void slbd_bbl(slbd_lattice *L, const slbd_params *P, int force) {
+  if (force) ensure_f(L);
+  if (L->prec == 32) launch_bbl<float>(L, P->moving != 0, force != 0);
+  else launch_bbl<double>(L, P->moving != 0, force != 0);
+}
+
+extern "C" void slbd_propagate(slbd_lattice *L) {
+  if (L->prec == 32)
+    hipLaunchKernelGGL(k_propagate<float>, plane_grid(L), dim3(256), 0, L->stream,
+                       (const float *)L->pops[L->cur], (float *)L->pops[L->cur ^ 1], L->G);
+  else
+    hipLaunchKernelGGL(k_propagate<double>, plane_grid(L), dim3(256), 0, L->stream,
+                       (const double *)L->pops[L->cur], (double *)L->pops[L->cur ^ 1], L->G);
+  LAUNCHCHECK();
+  L->cur ^= 1;
+}
+
+template <typename T>
+static void launch_collide(slbd_lattice *L, const PhysArgs &A, int bf) {
+  T *a = (T *)L->pops[L->cur];
+  const T *bx = (const T *)L->bx, *by = (const T *)L->by, *bz = (const T *)L->bz;
+  if (bf == 2)
+    hipLaunchKernelGGL((k_collide<T, 2>), lin_grid(L->N), dim3(256), 0, L->stream, a, L->flags, bx, by, bz, L->G, A);
+  else if (bf == 1)
+    hipLaunchKernelGGL((k_collide<T, 1>), lin_grid(L->N), dim3(256), 0, L->stream, a, L->flags, bx, by, bz, L->G, A);
+  else
+    hipLaunchKernelGGL((k_collide<T, 0>), lin_grid(L->N), dim3(256), 0, L->stream, a, L->flags, bx, by, bz, L->G, A);
+  LAUNCHCHECK();
+}
+
+extern "C" void slbd_collide(slbd_lattice *L, const slbd_params *P) {
+  const PhysArgs A = phys(P);
+  const int bf = bf_mode(L, P);
+  if (L->prec == 32) launch_collide<float>(L, A, bf);
+  else launch_collide<double>(L, A, bf);
+}
+
+static int red_blocks(long N) {
+  long b = (N + 255) / 256;
+  return (int)(b < RED_BLOCKS ? b : RED_BLOCKS);
+}
+
+template <typename T>
+static void launch_vmean(slbd_lattice *L, const PhysArgs &A, int bf, double fac, const double *psi) {
+  const int nb = red_blocks(L->N);
+  const T *a = (const T *)L->pops[L->cur];
+  const T *bx = (const T *)L->bx, *by = (const T *)L->by, *bz = (const T *)L->bz;
+  if (bf == 2)
+    hipLaunchKernelGGL((k_vmean<T, 2>), dim3(nb), dim3(256), 0, L->stream, a, L->flags, bx, by, bz, psi, fac, L->G, A, L->part);
+  else if (bf == 1)
+    hipLaunchKernelGGL((k_vmean<T, 1>), dim3(nb), dim3(256), 0, L->stream, a, L->flags, bx, by, bz, psi, fac, L->G, A, L->part);
+  else
+    hipLaunchKernelGGL((k_vmean<T, 0>), dim3(nb), dim3(256), 0, L->stream, a, L->flags, bx, by, bz, psi, fac, L->G, A, L->part);
+  LAUNCHCHECK();
+  hipLaunchKernelGGL(k_final<5>, dim3(1), dim3(256), 0, L->stream, L->part, nb, L->dout);
+  LAUNCHCHECK();
+}
+
+extern "C" void slbd_vmean(slbd_lattice *L, const slbd_params *P, double fac,
+                           const double *psi_host, double out[5]) {
+  const PhysArgs A = phys(P);
+  const int bf = bf_mode(L, P);
+  const double *psi = NULL;
+  if (psi_host) {
+    if (!L->psi) L->psi = (double *)dalloc(sizeof(double) * L->N);
+    copy_in(L->psi, psi_host, sizeof(double) * L->N, L->stream);
+    psi = L->psi;
+  }
+  if (L->prec == 32) launch_vmean<float>(L, A, bf, fac, psi);
+  else launch_vmean<double>(L, A, bf, fac, psi);
+  copy_out(out, L->dout, sizeof(double) * 5, L->stream);
+}
+
+template <typename T>
+static void launch_vgrab(slbd_lattice *L, const PhysArgs &A, int bf, double fac, double prefac, int zero) {
+  const T *a = (const T *)L->pops[L->cur];
+  const T *bx = (const T *)L->bx, *by = (const T *)L->by, *bz = (const T *)L->bz;
+  if (bf == 2)
+    hipLaunchKernelGGL((k_vgrab<T, 2>), lin_grid(L->N), dim3(256), 0, L->stream, a, L->flags, bx, by, bz, fac, prefac, L->G, A, L->ux, L->uy, L->uz, zero);
+  else if (bf == 1)
+    hipLaunchKernelGGL((k_vgrab<T, 1>), lin_grid(L->N), dim3(256), 0, L->stream, a, L->flags, bx, by, bz, fac, prefac, L->G, A, L->ux, L->uy, L->uz, zero);
+  else
+    hipLaunchKernelGGL((k_vgrab<T, 0>), lin_grid(L->N), dim3(256), 0, L->stream, a, L->flags, bx, by, bz, fac, prefac, L->G, A, L->ux, L->uy, L->uz, zero);
+  LAUNCHCHECK();
+}
+
+extern "C" void slbd_vgrab(slbd_lattice *L, const slbd_params *P, double fac, double prefac) {
+  const PhysArgs A = phys(P);
+  const int bf = bf_mode(L, P);
+  if (L->prec == 32) launch_vgrab<float>(L, A, bf, fac, prefac, 0);
+  else launch_vgrab<double>(L, A, bf, fac, prefac, 0);
+}
+
+extern "C" void slbd_vzero(slbd_lattice *L) {
+  slbd_params P;
+  memset(&P, 0, sizeof P);
+  P.tau = 1.0;
+  const PhysArgs A = phys(&P);
+  if (L->prec == 32) launch_vgrab<float>(L, A, 0, 0.0, 0.0, 1);
+  else launch_vgrab<double>(L, A, 0, 0.0, 0.0, 1);
+}
+
+static void put_tab(slbd_lattice *L, int nobj, const double *objtab) {
+  if (nobj > L->tab_cap) {
+    if (L->tab) HIPCHECK(hipFree(L->tab));
+    L->tab_cap = nobj + 16;
+    L->tab = (double *)dalloc(sizeof(double) * SLBD_OBJ_STRIDE * L->tab_cap);
+  }
+  if (nobj > 0) copy_in(L->tab, objtab, sizeof(double) * SLBD_OBJ_STRIDE * nobj, L->stream);
+}
+
+extern "C" void slbd_vspread(slbd_lattice *L, int nobj, const double *objtab, double rho0) {
+  put_tab(L, nobj, objtab);
+  hipLaunchKernelGGL(k_vspread, plane_grid(L), dim3(256), 0, L->stream, L->objidx, L->tab, rho0,
+                     L->ux, L->uy, L->uz, L->G);
+  LAUNCHCHECK();
+}
+
+extern "C" void slbd_distribf(slbd_lattice *L, int nobj, const double *objtab, double *frc,
+                              double *trq) {
+  if (nobj <= 0) return;
+  ensure_f(L);
+  put_tab(L, nobj, objtab);
+  if (nobj > L->acc_cap) {
+    if (L->acc) HIPCHECK(hipFree(L->acc));
+    L->acc_cap = nobj + 16;
+    L->acc = (double *)dalloc(sizeof(double) * 6 * L->acc_cap);
+  }
+  HIPCHECK(hipMemsetAsync(L->acc, 0, sizeof(double) * 6 * nobj, L->stream));
+  hipLaunchKernelGGL(k_distribf, dim3(red_blocks(L->N)), dim3(256), 0, L->stream, L->objidx,
+                     L->fx, L->fy, L->fz, L->tab, nobj, L->G, L->acc);
+  LAUNCHCHECK();
+  double *h = (double *)malloc(sizeof(double) * 6 * nobj);
+  copy_out(h, L->acc, sizeof(double) * 6 * nobj, L->stream);
+  for (int o = 0; o < nobj; o++) {
+    frc[3 * o] = h[6 * o]; frc[3 * o + 1] = h[6 * o + 1]; frc[3 * o + 2] = h[6 * o + 2];
+    trq[3 * o] = h[6 * o + 3]; trq[3 * o + 1] = h[6 * o + 4]; trq[3 * o + 2] = h[6 * o + 5];
+  }
+  free(h);
+}
+
+template <typename T>
+static void launch_residual(slbd_lattice *L, const PhysArgs &A, int bf, int nb) {
+  const T *a = (const T *)L->pops[L->cur];
+  const T *bx = (const T *)L->bx, *by = (const T *)L->by, *bz = (const T *)L->bz;
+  if (bf == 2)
+    hipLaunchKernelGGL((k_residual<T, 2>), dim3(nb), dim3(256), 0, L->stream, a, L->flags, bx, by, bz, L->G, A, L->uprev, L->part);
+  else if (bf == 1)
+    hipLaunchKernelGGL((k_residual<T, 1>), dim3(nb), dim3(256), 0, L->stream, a, L->flags, bx, by, bz, L->G, A, L->uprev, L->part);
+  else
+    hipLaunchKernelGGL((k_residual<T, 0>), dim3(nb), dim3(256), 0, L->stream, a, L->flags, bx, by, bz, L->G, A, L->uprev, L->part);
+  LAUNCHCHECK();
+}
+
+extern "C" void slbd_residual_reset(slbd_lattice *L) { L->uprev_valid = 0; }
+
+extern "C" double slbd_residual(slbd_lattice *L, const slbd_params *P) {
+  if (!L->uprev) {
+    L->uprev = (float *)dalloc(sizeof(float) * 3 * L->N);
+    L->uprev_valid = 0;
+  }
+  if (!L->uprev_valid) HIPCHECK(hipMemsetAsync(L->uprev, 0, sizeof(float) * 3 * L->N, L->stream));
+  const PhysArgs A = phys(P);
+  const int bf = bf_mode(L, P);
+  const int nb = red_blocks(L->N);
+  if (L->prec == 32) launch_residual<float>(L, A, bf, nb);
+  else launch_residual<double>(L, A, bf, nb);
+  hipLaunchKernelGGL(k_final<2>, dim3(1), dim3(256), 0, L->stream, L->part, nb, L->dout);
+  LAUNCHCHECK();
+  double o[2];
+  copy_out(o, L->dout, sizeof o, L->stream);
+  const int had = L->uprev_valid;
+  L->uprev_valid = 1;
+  if (!had) return -1.0;
+  return o[1] > 0.0 ? sqrt(o[0] / o[1]) : 0.0;
+}
+
+extern "C" void slbd_sum_pops(slbd_lattice *L, double out[15]) {
+  const int nb = red_blocks(L->N);
+  if (L->prec == 32)
+    hipLaunchKernelGGL(k_sum_pops<float>, dim3(nb), dim3(256), 0, L->stream,
+                       (const float *)L->pops[L->cur], L->flags, L->G, L->part);
+  else
+    hipLaunchKernelGGL(k_sum_pops<double>, dim3(nb), dim3(256), 0, L->stream,
+                       (const double *)L->pops[L->cur], L->flags, L->G, L->part);
+  LAUNCHCHECK();
+  hipLaunchKernelGGL(k_final<15>, dim3(1), dim3(256), 0, L->stream, L->part, nb, L->dout);
+  LAUNCHCHECK();
+  copy_out(out, L->dout, sizeof(double) * 15, L->stream);
+}
+
+/* ---- moment propagation ---- */
+
+static void ensure_mp_lattice(slbd_lattice *L) {
+  const size_t ts = tsize(L->prec);
+  if (!L->rho) {
+    L->rho = dalloc(L->N * ts);
+    HIPCHECK(hipMemsetAsync(L->rho, 0, L->N * ts, L->stream));
+  }
+  if (!L->mpf) {
+    L->mpf = (uint32_t *)dalloc(sizeof(uint32_t) * L->N);
+    hipLaunchKernelGGL(k_mpflags, plane_grid(L), dim3(256), 0, L->stream, L->objidx, L->mask,
+                       L->mpf, L->G);
+    LAUNCHCHECK();
+  }
+}
+
+extern "C" slbd_mp *slbd_mp_create(slbd_lattice *L, int nmom) {
+  ensure_mp_lattice(L);
+  slbd_mp *M = (slbd_mp *)calloc(1, sizeof(slbd_mp));
+  M->nmom = nmom > 1 ? 3 : 1;
+  M->prec = L->prec;
+  M->N = L->N;
+  const size_t ts = tsize(L->prec);
+  for (int f = 0; f < M->nmom; f++)
+    for (int b = 0; b < 2; b++) {
+      M->f[f][b] = dalloc(L->N * ts);
+      HIPCHECK(hipMemsetAsync(M->f[f][b], 0, L->N * ts, L->stream));
+    }
+  HIPCHECK(hipStreamSynchronize(L->stream));
+  return M;
+}
+
+extern "C" void slbd_mp_destroy(slbd_mp *M) {
+  if (!M) return;
+  HIPCHECK(hipDeviceSynchronize());
+  for (int f = 0; f < 3; f++)
+    for (int b = 0; b < 2; b++)
+      if (M->f[f][b]) HIPCHECK(hipFree(M->f[f][b]));
+  free(M);
+}
+
+static void check_mp(const slbd_lattice *L, const slbd_mp *M) {
+  if (M->prec != L->prec || M->N != L->N) {
+    fprintf(stderr, "sunlightlb: MP fields do not match the lattice (precision %d vs %d)\n",
+            M->prec, L->prec);
+    exit(EXIT_FAILURE);
+  }
+}
+
+extern "C" void slbd_mp_rhoinit(slbd_lattice *L, double rho0) {
+  ensure_mp_lattice(L);
+  if (L->prec == 32)
+    hipLaunchKernelGGL(k_mprho<float>, lin_grid(L->N), dim3(256), 0, L->stream,
+                       (const float *)L->pops[L->cur], L->flags, rho0, (float *)L->rho, L->G);
+  else
+    hipLaunchKernelGGL(k_mprho<double>, lin_grid(L->N), dim3(256), 0, L->stream,
+                       (const double *)L->pops[L->cur], L->flags, rho0, (double *)L->rho, L->G);
+  LAUNCHCHECK();
+}
+
+static void put_field_T(slbd_lattice *L, void *dev, const double *host) {
+  if (L->prec == 64) { copy_in(dev, host, sizeof(double) * L->N, L->stream); return; }
+  float *t = (float *)malloc(sizeof(float) * L->N);
+  for (long n = 0; n < L->N; n++) t[n] = (float)host[n];
+  copy_in(dev, t, sizeof(float) * L->N, L->stream);
+  free(t);
+}
+static void get_field_T(slbd_lattice *L, double *host, const void *dev) {
+  if (L->prec == 64) { copy_out(host, dev, sizeof(double) * L->N, L->stream); return; }
+  float *t = (float *)malloc(sizeof(float) * L->N);
+  copy_out(t, dev, sizeof(float) * L->N, L->stream);
+  for (long n = 0; n < L->N; n++) host[n] = (double)t[n];
+  free(t);
+}
+
+extern "C" void slbd_mp_put_rho(slbd_lattice *L, const double *rho) {
+  ensure_mp_lattice(L);
+  put_field_T(L, L->rho, rho);
+}
+extern "C" void slbd_mp_get_rho(slbd_lattice *L, double *rho) {
+  ensure_mp_lattice(L);
+  get_field_T(L, rho, L->rho);
+}
+extern "C" void slbd_mp_put(slbd_lattice *L, slbd_mp *M, int field, const double *v) {
+  check_mp(L, M);
+  if (field >= M->nmom) return;
+  put_field_T(L, M->f[field][M->cur], v);
+}
+extern "C" void slbd_mp_get(slbd_lattice *L, slbd_mp *M, int field, double *v) {
+  check_mp(L, M);
+  if (field >= M->nmom) { memset(v, 0, sizeof(double) * L->N); return; }
+  get_field_T(L, v, M->f[field][M->cur]);
+}
+extern "C" void slbd_mp_zero(slbd_lattice *L, slbd_mp *M, int field) {
+  check_mp(L, M);
+  if (field >= M->nmom) return;
+  HIPCHECK(hipMemsetAsync(M->f[field][M->cur], 0, L->N * tsize(L->prec), L->stream));
+}
+
+template <typename T, int NMOM, bool MOVING>
+static void launch_mp_t(slbd_lattice *L, slbd_mp *M, const MPArgs &A) {
+  const int c = M->cur;
+  hipLaunchKernelGGL((k_mpstep<T, NMOM, MOVING>), plane_grid(L), dim3(256), 0, L->stream,
+                     (const T *)L->pops[L->cur], (const T *)L->rho, (const T *)M->f[0][c],
+                     (T *)M->f[0][c ^ 1], (const T *)M->f[1][c], (T *)M->f[1][c ^ 1],
+                     (const T *)M->f[2][c], (T *)M->f[2][c ^ 1], L->mpf, L->mask, L->objidx,
+                     L->tab, L->G, A);
+  LAUNCHCHECK();
+  M->cur ^= 1;
+}
+
+template <typename T>
+static void launch_mp(slbd_lattice *L, slbd_mp *M, const MPArgs &A, bool moving) {
+  if (M->nmom > 1) {
+    if (moving) launch_mp_t<T, 3, true>(L, M, A); else launch_mp_t<T, 3, false>(L, M, A);
+  } else {
+    if (moving) launch_mp_t<T, 1, true>(L, M, A); else launch_mp_t<T, 1, false>(L, M, A);
+  }
+}
+
+extern "C" void slbd_mp_step(slbd_lattice *L, slbd_mp *M, double rho0, double alpha,
+                             int moving, int nobj, const double *objtab, double pads,
+                             int nmom, int axis, int nsteps) {
+  (void)nmom;
+  check_mp(L, M);
+  ensure_mp_lattice(L);
+  if (moving) put_tab(L, nobj, objtab);
+  MPArgs A;
+  A.rho0 = rho0; A.alpha = alpha; A.pads = pads; A.axis = axis;
+  for (int s = 0; s < nsteps; s++) {
+    if (L->prec == 32) launch_mp<float>(L, M, A, moving != 0);
+    else launch_mp<double>(L, M, A, moving != 0);
+  }
+}
+
+extern "C" double slbd_time_mp(slbd_lattice *L, slbd_mp *M, double rho0, double alpha,
+                               double pads, int nmom, int nsteps) {
+  HIPCHECK(hipStreamSynchronize(L->stream));
+  HIPCHECK(hipEventRecord(L->ev0, L->stream));
+  slbd_mp_step(L, M, rho0, alpha, 0, 0, NULL, pads, nmom, 0, nsteps);
+  HIPCHECK(hipEventRecord(L->ev1, L->stream));
+  HIPCHECK(hipEventSynchronize(L->ev1));
+  float ms = 0.0f;
+  HIPCHECK(hipEventElapsedTime(&ms, L->ev0, L->ev1));
+  return (double)ms;
+}
+
+extern "C" void slbd_mp_sums(slbd_lattice *L, slbd_mp *M, double out[6]) {
+  check_mp(L, M);
+  const int nb = red_blocks(L->N);
+  const int c = M->cur;
+  if (L->prec == 32)
+    hipLaunchKernelGGL(k_mpsums<float>, dim3(nb), dim3(256), 0, L->stream,
+                       (const float *)M->f[0][c], (const float *)M->f[1][c],
+                       (const float *)M->f[2][c], L->flags, L->mask, M->nmom, L->G, L->part);
+  else
+    hipLaunchKernelGGL(k_mpsums<double>, dim3(nb), dim3(256), 0, L->stream,
+                       (const double *)M->f[0][c], (const double *)M->f[1][c],
+                       (const double *)M->f[2][c], L->flags, L->mask, M->nmom, L->G, L->part);
+  LAUNCHCHECK();
+  hipLaunchKernelGGL(k_final<6>, dim3(1), dim3(256), 0, L->stream, L->part, nb, L->dout);
+  LAUNCHCHECK();
+  copy_out(out, L->dout, sizeof(double) * 6, L->stream);
+}
