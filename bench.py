@@ -82,6 +82,20 @@ def cpu_baseline(L, h, n, force, steps):
                        f"{dt:.1f} s")
 
 
+def traffic_bytes(n, precision):
+    """HBM bytes per k_step launch measured by rocprofv3 --pmc FETCH_SIZE and
+    WRITE_SIZE on this workload, each scaled by the factor the same counter
+    reads for a plain copy of known size (profiles/k_step_traffic.json)."""
+    path = os.path.join(ROOT, "profiles", "k_step_traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+        key = f"{n}_fp{precision}"
+        return t["workloads"][key]["traffic_bytes"] if key in t["workloads"] else None
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -95,6 +109,8 @@ def main():
     ap.add_argument("--precision", type=int, default=32, choices=(32, 64))
     ap.add_argument("--cpu-steps", type=int, default=4)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-test", action="store_true",
+                    help="only run the plain device copy kernel (counter calibration)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -106,6 +122,13 @@ def main():
     os.environ["SLB_PRECISION"] = str(args.precision)
     import sunlightlb_amd as s
     L = s.lib()
+    copy_bytes = 1 << 31
+    copy_ms = L.slb_hbm_copy_test(copy_bytes, 10)
+    copy_gbs = 2 * copy_bytes / (copy_ms * 1e-3) / 1e9
+    log(f"device copy: {copy_bytes >> 20} MiB in {copy_ms * 1e3:.1f} us, {copy_gbs:.0f} GB/s "
+        f"(read + write)")
+    if args.copy_test:
+        return
     n = args.size
     h, nsph, phi = build_pack(L, n, args.radius, args.phi, args.seed, args.force)
     N = n ** 3
@@ -150,9 +173,10 @@ def main():
             "peak": PEAK_HBM_GBS,
             "unit": "GB/s",
             "frac": round(achieved / PEAK_HBM_GBS, 4),
-            "traffic": None,
+            "traffic": traffic_bytes(n, args.precision),
             "kernel": "k_step (fused pull step)",
             "algorithmic_bytes_per_node": bytes_per_lup,
+            "device_copy_GBs": round(copy_gbs, 1),
         },
         "cpu_baseline": cpu,
     }

@@ -110,6 +110,11 @@ double slb_mp_time_steps(MPSys mp, int nsteps);
 
 /* ---- introspection ---- */
 
+/* Plain device copy of `bytes` (16 bytes per lane) repeated reps times:
+ * mean milliseconds per copy.  The achievable HBM copy rate of the device,
+ * and the known-size kernel against which memory counters are scaled. */
+double slb_hbm_copy_test(size_t bytes, int reps);
+
 /* Device memory held by the lattice, bytes. */
 size_t slb_device_bytes(LBSys lb);
 /* GPU architecture name, e.g. "gfx950". */

@@ -157,6 +157,7 @@ SIGNATURES = {
     "slb_mp_sums": (None, [MPSys, RealP]), "slb_mp_run": (None, [MPSys, I]),
     "slb_mp_time_steps": (D, [MPSys, I]),
     "slb_device_bytes": (C.c_size_t, [LBSys]), "slb_device_arch": (C.c_char_p, []),
+    "slb_hbm_copy_test": (D, [C.c_size_t, I]),
 }
 
 GLOBALS = ("slb_verbosity", "slb_signal_flag")

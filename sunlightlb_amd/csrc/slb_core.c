@@ -636,3 +636,8 @@ void slb_sum_pops(LBSys lb, double out[15]) {
 
 size_t slb_device_bytes(LBSys lb) { return slbd_bytes(PRIV(lb)->dev); }
 const char *slb_device_arch(void) { return slbd_arch(); }
+
+double slb_hbm_copy_test(size_t bytes, int reps) {
+  if (slbd_device_count() < 1) error("no HIP device is visible");
+  return slbd_copy_test(bytes, reps);
+}

@@ -119,6 +119,9 @@ double slbd_time_steps(slbd_lattice *L, const slbd_params *P, int nsteps);
 double slbd_time_mp(slbd_lattice *L, slbd_mp *M, double rho0, double alpha,
                     double pads, int nmom, int nsteps);
 size_t slbd_bytes(const slbd_lattice *L);
+/* Plain 16-byte-per-lane copy of `bytes` between two device buffers, reps
+ * times; mean milliseconds per copy (HBM roofline and counter calibration). */
+double slbd_copy_test(size_t bytes, int reps);
 
 #ifdef __cplusplus
 }

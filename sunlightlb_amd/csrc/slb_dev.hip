@@ -98,7 +98,27 @@ __device__ __forceinline__ long nb_off(const AxisOff &oi, const AxisOff &oj, con
 struct Geom {
   int esi, esj, esk, plane;
   long N;
+  int xcd_map;   /* step kernel: give each XCD a contiguous range of rows */
+  int nt_store;  /* step kernel: non-temporal population stores */
 };
+
+/* Block coordinates of the step kernel.  Workgroups are dispatched round
+ * robin over the 8 XCDs; with xcd_map the linear block id b is served by
+ * XCD b % 8 and that XCD walks a contiguous 1/8 of the (row, plane) space,
+ * so blocks that share boundary rows share an L2. */
+__device__ __forceinline__ void step_block(const Geom &G, int &bx, int &by) {
+  bx = blockIdx.x;
+  by = blockIdx.y;
+  if (G.xcd_map) {
+    const int nbx = gridDim.x, nb = gridDim.x * gridDim.y;
+    if ((nb & 7) == 0) {
+      const int lin = by * nbx + bx;
+      const int lin2 = (lin & 7) * (nb >> 3) + (lin >> 3);
+      bx = lin2 % nbx;
+      by = lin2 / nbx;
+    }
+  }
+}
 
 /* Compile-time unrolled loop over links 1..14 */
 template <int L, int END>
@@ -196,9 +216,11 @@ k_step(const T *__restrict__ src, T *__restrict__ dst,
        const T *__restrict__ bx, const T *__restrict__ by, const T *__restrict__ bz,
        double *__restrict__ fx, double *__restrict__ fy, double *__restrict__ fz,
        Geom G, PhysArgs P) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  int blk_x, blk_y;
+  step_block(G, blk_x, blk_y);
+  const int t = blk_x * blockDim.x + threadIdx.x;
   if (t >= G.plane) return;
-  const int i = blockIdx.y;
+  const int i = blk_y;
   const int j = t / G.esk;
   const int k = t - j * G.esk;
   const long N = G.N;
@@ -252,8 +274,13 @@ k_step(const T *__restrict__ src, T *__restrict__ dst,
     }
     collide_node<T>(v, jbx, jby, jbz, P);
   }
+  if (G.nt_store) {
 #pragma unroll
-  for (int l = 0; l < 15; l++) dst[(long)l * N + x] = v[l];
+    for (int l = 0; l < 15; l++) __builtin_nontemporal_store(v[l], &dst[(long)l * N + x]);
+  } else {
+#pragma unroll
+    for (int l = 0; l < 15; l++) dst[(long)l * N + x] = v[l];
+  }
 }
 
 /* ---------------------------------------------------------------------- */
@@ -882,6 +909,8 @@ extern "C" slbd_lattice *slbd_create(int esi, int esj, int esk, int prec) {
   L->esi = esi; L->esj = esj; L->esk = esk; L->prec = prec;
   L->N = (long)esi * esj * esk;
   L->G.esi = esi; L->G.esj = esj; L->G.esk = esk; L->G.plane = esj * esk; L->G.N = L->N;
+  L->G.xcd_map = getenv("SLB_XCD_MAP") ? atoi(getenv("SLB_XCD_MAP")) : 0;
+  L->G.nt_store = getenv("SLB_NT_STORE") ? atoi(getenv("SLB_NT_STORE")) : 1;
   HIPCHECK(hipStreamCreateWithFlags(&L->stream, hipStreamNonBlocking));
   HIPCHECK(hipEventCreate(&L->ev0));
   HIPCHECK(hipEventCreate(&L->ev1));
@@ -1555,4 +1584,38 @@ extern "C" void slbd_mp_sums(slbd_lattice *L, slbd_mp *M, double out[6]) {
   hipLaunchKernelGGL(k_final<6>, dim3(1), dim3(256), 0, L->stream, L->part, nb, L->dout);
   LAUNCHCHECK();
   copy_out(out, L->dout, sizeof(double) * 6, L->stream);
+}
+
+/* ---- device copy bandwidth (roofline and counter calibration) ---- */
+
+__global__ void __launch_bounds__(256)
+k_copy16(const uint4 *__restrict__ s, uint4 *__restrict__ d, long n) {
+  const long x = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (x < n) d[x] = s[x];
+}
+
+/* Copy `bytes` (rounded down to 16) between two fresh device buffers reps
+ * times; returns the mean milliseconds per copy. */
+extern "C" double slbd_copy_test(size_t bytes, int reps) {
+  const long n = (long)(bytes / 16);
+  if (n <= 0 || reps <= 0) return 0.0;
+  uint4 *a = (uint4 *)dalloc((size_t)n * 16), *b = (uint4 *)dalloc((size_t)n * 16);
+  HIPCHECK(hipMemset(a, 0, (size_t)n * 16));
+  hipEvent_t e0, e1;
+  HIPCHECK(hipEventCreate(&e0));
+  HIPCHECK(hipEventCreate(&e1));
+  const unsigned g = (unsigned)((n + 255) / 256);
+  hipLaunchKernelGGL(k_copy16, dim3(g), dim3(256), 0, 0, a, b, n);
+  HIPCHECK(hipEventRecord(e0, 0));
+  for (int r = 0; r < reps; r++) hipLaunchKernelGGL(k_copy16, dim3(g), dim3(256), 0, 0, a, b, n);
+  HIPCHECK(hipEventRecord(e1, 0));
+  HIPCHECK(hipEventSynchronize(e1));
+  LAUNCHCHECK();
+  float ms = 0.0f;
+  HIPCHECK(hipEventElapsedTime(&ms, e0, e1));
+  HIPCHECK(hipEventDestroy(e0));
+  HIPCHECK(hipEventDestroy(e1));
+  HIPCHECK(hipFree(a));
+  HIPCHECK(hipFree(b));
+  return (double)ms / reps;
 }
