@@ -96,6 +96,92 @@ def traffic_bytes(n, precision):
         return None
 
 
+def converge(L, args):
+    """Cold-start solve of the pack to the velocity residual tolerance in FP64
+    and in FP32; permeability k = vmean eta / F (examples/rcp_perm.pl:161-167)
+    from measure() at the end of each run."""
+    res = {"tol": args.tol, "interval": args.interval}
+    eta = 72.0 * (0.75 - 0.5) / 3.0
+    for prec in (64, 32):
+        os.environ["SLB_PRECISION"] = str(prec)
+        h, _, _ = build_pack(L, args.size, args.radius, args.phi, args.seed, args.force)
+        r = C.c_double(0.0)
+        t0 = time.perf_counter()
+        steps = L.slb_run_to_residual(h, args.tol, args.interval, args.maxsteps, C.byref(r))
+        L.measure(h)
+        dt = time.perf_counter() - t0
+        k = h.contents.vmeanx * eta / args.force
+        log(f"converge fp{prec}: {steps} steps to residual {r.value:.3e} in {dt:.1f} s, "
+            f"k = {k:.10g}")
+        res[f"fp{prec}"] = {"steps": steps, "residual": r.value, "seconds": round(dt, 3),
+                            "permeability": k, "vmeanx": h.contents.vmeanx}
+        L.slb_destroy(h)
+    k64, k32 = res["fp64"]["permeability"], res["fp32"]["permeability"]
+    res["permeability_rel_diff_fp32_vs_fp64"] = abs(k32 - k64) / abs(k64)
+    return res
+
+
+def mp_workload(L, args, n=128, R=6.0, phi=0.25, peclet=10.0, alpha=0.0, pads=0.01):
+    """Passive-scalar moment propagation on the steady flow of a sphere pack.
+
+    Steady Stokes flow of the pack (cold start to the residual tolerance), the
+    populations scaled so that Pe = U (2R) / D = peclet with U the
+    interstitial mean speed (superficial vmean / porosity) and D = (1 -
+    alpha)/6; uniform unit tracer on fluid nodes, partial adsorption pads at
+    boundary solid nodes, displacement moments along x, mp_steps steps in the
+    lattice precision with FP64 moment sums."""
+    os.environ["SLB_PRECISION"] = str(args.precision)
+    h, nsph, phi_got = build_pack(L, n, R, phi, args.seed + 1, 1e-6)
+    r = C.c_double(0.0)
+    steps = L.slb_run_to_residual(h, args.tol, args.interval, args.maxsteps, C.byref(r))
+    L.measure(h)
+    nfluid = L.countfluid(h)
+    porosity = nfluid / float(n ** 3)
+    U = h.contents.vmeanx / porosity
+    D = (1.0 - alpha) / 6.0
+    target = peclet * D / (2.0 * R)
+    L.slb_scale_pops(h, target / U)
+    mp = L.newmp(h, b"null")
+    mp.contents.alpha = alpha
+    L.mpinit(mp)
+    L.mpall(mp, 1.0)
+    L.slb_mp_set_adsorption(mp, pads)
+    L.slb_mp_track_moments(mp, 0)
+    s0 = (C.c_double * 6)()
+    L.slb_mp_sums(mp, s0)
+    L.slb_mp_run(mp, 10)
+    L.mpinit(mp)
+    L.mpall(mp, 1.0)
+    L.slb_mp_track_moments(mp, 0)
+    ms = L.slb_mp_time_steps(mp, args.mp_steps)
+    s1 = (C.c_double * 6)()
+    L.slb_mp_sums(mp, s1)
+    ptot = s1[0]
+    mean = s1[3] / ptot
+    var = s1[4] / ptot - mean * mean
+    N = n ** 3
+    mlups = N * args.mp_steps / (ms * 1e-3) / 1e6
+    bpl = (15 + 1 + 3 * 2) * (4 if args.precision == 32 else 8) + 4
+    res = {
+        "workload": f"{n}^3 RSA pack R {R:g} phi {phi_got:.4f} ({nsph} spheres), flow solved "
+                    f"to residual {r.value:.2e} in {steps} steps, scaled to Pe {peclet:g} "
+                    f"(U {target:.5g}, D {D:.5g}), alpha {alpha:g}, adsorption {pads:g}",
+        "steps": args.mp_steps,
+        "surviving_fraction": ptot / s0[0],
+        "adsorbed_fraction": (s1[1]) / s0[0],
+        "mean_displacement_x": mean,
+        "variance_displacement_x": var,
+        "expected_mean_no_adsorption": target * args.mp_steps,
+        "ms_per_step": ms / args.mp_steps,
+        "MLUPS": round(mlups, 1),
+        "algorithmic_bytes_per_node": bpl,
+        "achieved_GBs": round(bpl * N / (ms / args.mp_steps * 1e-3) / 1e9, 1),
+    }
+    log(f"mp: {res}")
+    L.slb_destroy(h)
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -109,6 +195,16 @@ def main():
     ap.add_argument("--precision", type=int, default=32, choices=(32, 64))
     ap.add_argument("--cpu-steps", type=int, default=4)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--converge", action="store_true",
+                    help="also run FP64 and FP32 to the velocity residual tolerance and "
+                         "compare permeabilities")
+    ap.add_argument("--tol", type=float, default=1e-6)
+    ap.add_argument("--interval", type=int, default=100)
+    ap.add_argument("--maxsteps", type=int, default=200000)
+    ap.add_argument("--mp", action="store_true",
+                    help="also run the passive-scalar workload: 128^3 pack (R 6, phi 0.25) "
+                         "at Peclet 10, adsorption 0.01, 5000 moment-propagation steps")
+    ap.add_argument("--mp-steps", type=int, default=5000)
     ap.add_argument("--copy-test", action="store_true",
                     help="only run the plain device copy kernel (counter calibration)")
     args = ap.parse_args()
@@ -145,6 +241,10 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(L, h, n, args.force, args.cpu_steps)
+    conv = None
+    if args.converge:
+        L.slb_destroy(h)
+        conv = converge(L, args)
 
     dtype = "f32" if args.precision == 32 else "f64"
     out = {
@@ -180,6 +280,10 @@ def main():
         },
         "cpu_baseline": cpu,
     }
+    if conv is not None:
+        out["converge"] = conv
+    if args.mp:
+        out["mp"] = mp_workload(L, args)
     print(json.dumps(out), flush=True)
 
 

@@ -37,6 +37,11 @@ void slb_sync_device(LBSys lb);
 void slb_mp_sync_host(MPSys mp);
 void slb_mp_sync_device(MPSys mp);
 
+/* Release a lattice created by newlb(): its objects, its MP systems and all
+ * device memory (the reference API has no destructor).  Every handle
+ * derived from lb is invalid afterwards. */
+void slb_destroy(LBSys lb);
+
 /* ---- stepping ---- */
 
 /* nsteps lbstep() calls without host round trips (the step counter, status

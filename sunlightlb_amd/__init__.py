@@ -146,6 +146,7 @@ SIGNATURES = {
     "slb_set_precision": (I, [LBSys, I]), "slb_get_precision": (I, [LBSys]),
     "slb_sync_host": (None, [LBSys]), "slb_sync_device": (None, [LBSys]),
     "slb_mp_sync_host": (None, [MPSys]), "slb_mp_sync_device": (None, [MPSys]),
+    "slb_destroy": (None, [LBSys]),
     "slb_run": (None, [LBSys, I]), "slb_residual": (D, [LBSys]),
     "slb_residual_reset": (None, [LBSys]),
     "slb_run_to_residual": (I, [LBSys, D, I, I, RealP]),
@@ -245,6 +246,11 @@ class Lattice:
             out.append(p.contents)
             p = p.contents.next_obj
         return out
+
+    def free(self):
+        if self.h:
+            self.L.slb_destroy(self.h)
+            self.h = None
 
     def step(self, n=1):
         if n == 1:

@@ -641,3 +641,33 @@ double slb_hbm_copy_test(size_t bytes, int reps) {
   if (slbd_device_count() < 1) error("no HIP device is visible");
   return slbd_copy_test(bytes, reps);
 }
+
+/* Release a lattice, its objects, its MP systems and all device memory.
+ * The handle (and every MPSys, MPIso and object pointer of it) is invalid
+ * afterwards. */
+void slb_destroy(LBSys lb) {
+  if (lb == NULL) return;
+  slb_lbpriv *S = PRIV(lb);
+  for (slb_mpwrap *m = S->mps; m;) {
+    slb_mpwrap *next = m->priv.next;
+    slbd_mp_destroy(m->priv.dev);
+    free(m->pub.p);
+    free(m->pub.newp);
+    free(m->pub.name);
+    free(m);
+    m = next;
+  }
+  slbd_destroy(S->dev);
+  for (LBObj p = lb->obj_head; p != NULL;) {
+    LBObj q = p->next_obj;
+    free(p);
+    p = q;
+  }
+  free(lb->c); free(lb->w); free(lb->a); free(lb->mask); free(lb->obj);
+  free(lb->ux); free(lb->uy); free(lb->uz);
+  free(lb->fx); free(lb->fy); free(lb->fz);
+  free(lb->bx); free(lb->by); free(lb->bz);
+  free(lb->rho); free(lb->name);
+  free(S->objlist); free(S->objtab); free(S->objidx);
+  free((slb_lbwrap *)lb);
+}

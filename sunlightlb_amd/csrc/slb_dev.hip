@@ -581,7 +581,7 @@ template <typename T, int BF>
 __global__ void __launch_bounds__(256)
 k_residual(const T *__restrict__ a, const uint16_t *__restrict__ flags,
            const T *__restrict__ bx, const T *__restrict__ by, const T *__restrict__ bz,
-           Geom G, PhysArgs P, float *__restrict__ up, double *__restrict__ part) {
+           Geom G, PhysArgs P, double *__restrict__ up, double *__restrict__ part) {
   double s[2] = {0.0, 0.0};
   const long N = G.N;
   for (long x = (long)blockIdx.x * blockDim.x + threadIdx.x; x < N;
@@ -599,7 +599,7 @@ k_residual(const T *__restrict__ a, const uint16_t *__restrict__ flags,
     const double e0 = u0 - up[x], e1 = u1 - up[N + x], e2 = u2 - up[2 * N + x];
     s[0] += e0 * e0 + e1 * e1 + e2 * e2;
     s[1] += u0 * u0 + u1 * u1 + u2 * u2;
-    up[x] = (float)u0; up[N + x] = (float)u1; up[2 * N + x] = (float)u2;
+    up[x] = u0; up[N + x] = u1; up[2 * N + x] = u2;
   }
   block_reduce_store<2>(s, part);
 }
@@ -857,7 +857,7 @@ struct slbd_lattice {
   int tab_cap;
   double *acc;              /* distribf accumulators */
   int acc_cap;
-  float *uprev;             /* residual previous velocity */
+  double *uprev;            /* residual previous velocity */
   int uprev_valid;
   /* moment propagation: frozen density and link flags (per lattice) */
   void *rho;
@@ -1388,10 +1388,10 @@ extern "C" void slbd_residual_reset(slbd_lattice *L) { L->uprev_valid = 0; }
 
 extern "C" double slbd_residual(slbd_lattice *L, const slbd_params *P) {
   if (!L->uprev) {
-    L->uprev = (float *)dalloc(sizeof(float) * 3 * L->N);
+    L->uprev = (double *)dalloc(sizeof(double) * 3 * L->N);
     L->uprev_valid = 0;
   }
-  if (!L->uprev_valid) HIPCHECK(hipMemsetAsync(L->uprev, 0, sizeof(float) * 3 * L->N, L->stream));
+  if (!L->uprev_valid) HIPCHECK(hipMemsetAsync(L->uprev, 0, sizeof(double) * 3 * L->N, L->stream));
   const PhysArgs A = phys(P);
   const int bf = bf_mode(L, P);
   const int nb = red_blocks(L->N);
