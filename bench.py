@@ -182,6 +182,137 @@ def mp_workload(L, args, n=128, R=6.0, phi=0.25, peclet=10.0, alpha=0.0, pads=0.
     return res
 
 
+def make_boxes(nbox, seed):
+    """Seeded boxes: cubic edge U{32..64}, 2..20 non-overlapping spheres of
+    radius U[3, 6] (minimum-image gap of one lattice unit); sphere 0 is the
+    one that moves."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    boxes = []
+    for _ in range(nbox):
+        Lb = int(rng.integers(32, 65))
+        ns = int(rng.integers(2, 21))
+        sph = []
+        for _t in range(20000):
+            if len(sph) == ns:
+                break
+            R = float(rng.uniform(3.0, 6.0))
+            c = rng.uniform(0.0, Lb, 3)
+            ok = True
+            for (R2, *c2) in sph:
+                d = c - np.array(c2)
+                d -= Lb * np.round(d / Lb)
+                if np.sqrt((d * d).sum()) < R + R2 + 1.0:
+                    ok = False
+                    break
+            if ok:
+                sph.append((R, *c))
+        boxes.append((Lb, sph))
+    return boxes
+
+
+def insert_periodic(L, h, Lb, R, c):
+    """Insert a sphere and every periodic image whose bounding box meets the
+    box, each image its own object centred on itself (correct lever arms
+    for the torque, which the reference sums without minimum image)."""
+    objs = []
+    for a in (-1, 0, 1):
+        for b_ in (-1, 0, 1):
+            for e in (-1, 0, 1):
+                o = (c[0] + a * Lb, c[1] + b_ * Lb, c[2] + e * Lb)
+                if all(-R < o[d] < Lb + R for d in range(3)):
+                    p = L.insertsph(h, R, *o)
+                    if p:
+                        objs.append(C.addressof(p.contents))
+    return objs
+
+
+def batch_workload(L, args, u0=1e-3, tau=1.0):
+    """512 independent periodic boxes, each with 6 right-hand sides (unit
+    translation and rotation of sphere 0, the other spheres fixed), stepped
+    together: one launch per step over all problems, autotuned launch shape,
+    HIP-graph replay.  Reports throughput and the 6x6 resistance matrices of
+    sphere 0 after measure()."""
+    import numpy as np
+    prec = args.precision
+    os.environ["SLB_PRECISION"] = str(prec)
+    t0 = time.perf_counter()
+    boxes = make_boxes(args.batch_boxes, args.seed + 7)
+    b = L.slb_batch_new(prec)
+    moving_rows = []
+    for (Lb, sph) in boxes:
+        h = L.newlb(Lb, Lb, Lb, b"null")
+        h.contents.tau, h.contents.stokes_flow, h.contents.objects_moving = tau, 1, 1
+        mover = []
+        for q in range(len(sph) - 1, -1, -1):   # sphere 0 inserted last: head of the list
+            objs = insert_periodic(L, h, Lb, sph[q][0], sph[q][1:])
+            if q == 0:
+                mover = objs
+        L.syncobjs(h)
+        order = []
+        p = h.contents.obj_head
+        while p:
+            order.append(C.addressof(p.contents))
+            p = p.contents.next_obj
+        rows = [order.index(a) for a in mover]
+        vel = np.zeros((6, len(order), 6))
+        R0 = sph[0][0]
+        for k in range(6):
+            amp = u0 if k < 3 else u0 / R0
+            vel[k, rows, k] = amp
+        flat = np.ascontiguousarray(vel.reshape(-1))
+        L.slb_batch_add(b, h, 6, flat.ctypes.data_as(C.POINTER(C.c_double)))
+        moving_rows.append((rows, len(order), R0))
+        L.slb_destroy(h)
+    L.slb_batch_commit(b)
+    setup = time.perf_counter() - t0
+    t1 = time.perf_counter()
+    L.slb_batch_autotune(b, None)
+    tune = time.perf_counter() - t1
+    cfg = (C.c_int * 4)()
+    L.slb_batch_get_config(b, cfg)
+    nodes = L.slb_batch_nodes(b)
+    L.slb_batch_run(b, 32)
+    ms = L.slb_batch_time_steps(b, args.batch_steps)
+    L.slb_batch_measure(b)
+    mlups = nodes * args.batch_steps / (ms * 1e-3) / 1e6
+    bpl = 2 * 15 * (4 if prec == 32 else 8)
+    eta = 72.0 * (tau - 0.5) / 3.0
+    drag, asym = [], []
+    for q, (rows, nobj, R0) in enumerate(moving_rows):
+        Rm = np.zeros((6, 6))
+        for k in range(6):
+            frc = np.zeros(3 * nobj)
+            trq = np.zeros(3 * nobj)
+            L.slb_batch_forces(b, q, k, frc.ctypes.data_as(C.POINTER(C.c_double)),
+                               trq.ctypes.data_as(C.POINTER(C.c_double)))
+            F = frc.reshape(nobj, 3)[rows].sum(0)
+            T = trq.reshape(nobj, 3)[rows].sum(0)
+            amp = u0 if k < 3 else u0 / R0
+            Rm[:, k] = np.concatenate([F, T]) / amp
+        drag.append(-np.mean(np.diag(Rm)[:3]) / (6 * np.pi * eta * R0))
+        asym.append(np.abs(Rm - Rm.T).max() / np.abs(Rm).max())
+    L.slb_batch_free(b)
+    res = {
+        "workload": f"{args.batch_boxes} periodic boxes, edge U{{32..64}}, 2-20 spheres radius "
+                    f"U[3,6] (seed {args.seed + 7}), sphere 0 translating/rotating (6 rhs), "
+                    f"Stokes, tau {tau:g}, {'f32' if prec == 32 else 'f64'}",
+        "problems": 6 * args.batch_boxes,
+        "nodes_per_step": nodes,
+        "steps": args.batch_steps,
+        "ms_per_step": round(ms / args.batch_steps, 4),
+        "MLUPS": round(mlups, 1),
+        "achieved_GBs": round(bpl * nodes / (ms / args.batch_steps * 1e-3) / 1e9, 1),
+        "launch": {"block": cfg[0], "nodes_per_thread": cfg[1], "grid": cfg[2],
+                   "tiles": cfg[3], "autotune_s": round(tune, 2)},
+        "setup_s": round(setup, 1),
+        "translational_drag_over_stokes_median": float(np.median(drag)),
+        "resistance_asymmetry_median": float(np.median(asym)),
+    }
+    log(f"batch: {res}")
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -205,6 +336,11 @@ def main():
                     help="also run the passive-scalar workload: 128^3 pack (R 6, phi 0.25) "
                          "at Peclet 10, adsorption 0.01, 5000 moment-propagation steps")
     ap.add_argument("--mp-steps", type=int, default=5000)
+    ap.add_argument("--batch", action="store_true",
+                    help="also run the batched resistance-matrix workload (512 boxes x 6 "
+                         "right-hand sides in one launch per step)")
+    ap.add_argument("--batch-boxes", type=int, default=512)
+    ap.add_argument("--batch-steps", type=int, default=200)
     ap.add_argument("--copy-test", action="store_true",
                     help="only run the plain device copy kernel (counter calibration)")
     args = ap.parse_args()
@@ -284,6 +420,8 @@ def main():
         out["converge"] = conv
     if args.mp:
         out["mp"] = mp_workload(L, args)
+    if args.batch:
+        out["batch"] = batch_workload(L, args)
     print(json.dumps(out), flush=True)
 
 

@@ -113,6 +113,51 @@ void slb_mp_run(MPSys mp, int nsteps);
  * advances).  Returns milliseconds. */
 double slb_mp_time_steps(MPSys mp, int nsteps);
 
+/* ---- batched independent boxes (resistance problems) ---- */
+
+/* A batch of independent periodic boxes, each with nrhs right-hand sides
+ * (object velocity sets), stepped together: one kernel launch per step over
+ * all problems, the step loop replayed from a captured HIP graph. */
+typedef struct slb_batch_s *SLBBatch;
+
+/* New empty batch; prec 32 or 64 (populations; force sums are FP64). */
+SLBBatch slb_batch_new(int prec);
+
+/* Add lb's box: its solid map and boundary-link mask (lb is synchronised),
+ * tau, stokes_flow, skip_solid and uniform body force, as nrhs problems
+ * starting from rest.  objvel holds nrhs x nobj rows of (vx, vy, vz, wx, wy,
+ * wz) for lb's objects in list order (NULL: every rhs uses the objects' own
+ * velx..angz); wall velocities follow vspread() (rho0 (v + w x r)/12, r from
+ * the object centre).  lb itself is not changed and may be destroyed after
+ * the call.  Returns the box index, or -1. */
+int  slb_batch_add(SLBBatch b, LBSys lb, int nrhs, const double *objvel);
+
+/* Allocate the device arenas and the tile schedule; no more boxes after. */
+int  slb_batch_commit(SLBBatch b);
+
+/* Choose block size, nodes per thread and persistent grid size by timing
+ * candidate shapes on this batch (before its first step; the state is reset
+ * to rest afterwards).  With cache_path, a shape class already in the file is
+ * reused without timing, and new results are appended.  Returns 1 on a cache
+ * hit, 0 after timing, -1 on error. */
+int  slb_batch_autotune(SLBBatch b, const char *cache_path);
+void slb_batch_set_config(SLBBatch b, int block, int nodes_per_thread, int grid);
+/* cfg = {block, nodes per thread, grid (0 = one block per tile), tiles} */
+void slb_batch_get_config(SLBBatch b, int cfg[4]);
+
+void   slb_batch_run(SLBBatch b, int nsteps);
+double slb_batch_time_steps(SLBBatch b, int nsteps);
+/* measure(): two force-collecting steps, force and torque per object halved
+ * (FP64 sums, torque about each object's own centre, no minimum image). */
+void slb_batch_measure(SLBBatch b);
+/* Force and torque (3 each per object, list order) of one problem after
+ * slb_batch_measure(); returns the object count or -1. */
+int  slb_batch_forces(SLBBatch b, int box, int rhs, double *frc, double *trq);
+/* Populations of one problem, AoS a[15n+l] as lb->a. */
+int  slb_batch_get_pops(SLBBatch b, int box, int rhs, double *aos);
+long slb_batch_nodes(SLBBatch b);
+void slb_batch_free(SLBBatch b);
+
 /* ---- introspection ---- */
 
 /* Plain device copy of `bytes` (16 bytes per lane) repeated reps times:

@@ -159,6 +159,19 @@ SIGNATURES = {
     "slb_mp_time_steps": (D, [MPSys, I]),
     "slb_device_bytes": (C.c_size_t, [LBSys]), "slb_device_arch": (C.c_char_p, []),
     "slb_hbm_copy_test": (D, [C.c_size_t, I]),
+    "slb_batch_new": (C.c_void_p, [I]),
+    "slb_batch_add": (I, [C.c_void_p, LBSys, I, RealP]),
+    "slb_batch_commit": (I, [C.c_void_p]),
+    "slb_batch_autotune": (I, [C.c_void_p, S]),
+    "slb_batch_set_config": (None, [C.c_void_p, I, I, I]),
+    "slb_batch_get_config": (None, [C.c_void_p, IntP]),
+    "slb_batch_run": (None, [C.c_void_p, I]),
+    "slb_batch_time_steps": (D, [C.c_void_p, I]),
+    "slb_batch_measure": (None, [C.c_void_p]),
+    "slb_batch_forces": (I, [C.c_void_p, I, I, RealP, RealP]),
+    "slb_batch_get_pops": (I, [C.c_void_p, I, I, RealP]),
+    "slb_batch_nodes": (C.c_long, [C.c_void_p]),
+    "slb_batch_free": (None, [C.c_void_p]),
 }
 
 GLOBALS = ("slb_verbosity", "slb_signal_flag")

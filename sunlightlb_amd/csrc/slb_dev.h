@@ -123,6 +123,29 @@ size_t slbd_bytes(const slbd_lattice *L);
  * times; mean milliseconds per copy (HBM roofline and counter calibration). */
 double slbd_copy_test(size_t bytes, int reps);
 
+/* ---- batched independent lattices (slb_batch.hip) ---- */
+typedef struct slbd_batch slbd_batch;
+slbd_batch *slbd_batch_create(int prec);
+int  slbd_batch_add_box(slbd_batch *B, int esi, int esj, int esk, const int32_t *objidx,
+                        const int32_t *mask, int nobj, int nrhs, const double *tabs,
+                        double tau, double rho0, int stokes, int skip_solid, int moving,
+                        const double *bf);
+int  slbd_batch_commit(slbd_batch *B);
+void slbd_batch_config(slbd_batch *B, int block, int npt, int grid);
+void slbd_batch_get_config(const slbd_batch *B, int *cfg);
+void slbd_batch_run(slbd_batch *B, int nsteps, int use_graph);
+double slbd_batch_time(slbd_batch *B, int nsteps, int use_graph);
+void slbd_batch_measure(slbd_batch *B);
+int  slbd_batch_forces(slbd_batch *B, int box, int rhs, double *frc, double *trq);
+int  slbd_batch_get_pops(slbd_batch *B, int box, int rhs, double *aos);
+long slbd_batch_nodes(const slbd_batch *B);
+int  slbd_batch_nprob(const slbd_batch *B);
+int  slbd_batch_nbox(const slbd_batch *B);
+int  slbd_batch_box_dims(const slbd_batch *B, int box, int *dims);
+int  slbd_batch_cu_count(void);
+void slbd_batch_reset(slbd_batch *B);
+void slbd_batch_destroy(slbd_batch *B);
+
 #ifdef __cplusplus
 }
 #endif

@@ -36,173 +36,7 @@
 #include <string.h>
 #include "slb_dev.h"
 
-#define HIPCHECK(call)                                                        \
-  do {                                                                        \
-    hipError_t e_ = (call);                                                   \
-    if (e_ != hipSuccess) {                                                   \
-      fprintf(stderr, "sunlightlb: HIP error '%s' (%d) at %s:%d in %s\n",     \
-              hipGetErrorString(e_), (int)e_, __FILE__, __LINE__, #call);     \
-      fflush(stderr);                                                         \
-      exit(EXIT_FAILURE);                                                     \
-    }                                                                         \
-  } while (0)
-
-#define LAUNCHCHECK() HIPCHECK(hipGetLastError())
-
-/* ---------------------------------------------------------------------- */
-/* D3Q15 tables (the reference's src/lbmisc.c:89-104 ordering).            */
-
-__host__ __device__ constexpr int CX(int l) {
-  return (l == 1 || l == 7 || l == 9 || l == 11 || l == 13) ? 1
-       : (l == 2 || l == 8 || l == 10 || l == 12 || l == 14) ? -1 : 0;
-}
-__host__ __device__ constexpr int CY(int l) {
-  return (l == 3 || l == 7 || l == 8 || l == 11 || l == 12) ? 1
-       : (l == 4 || l == 9 || l == 10 || l == 13 || l == 14) ? -1 : 0;
-}
-__host__ __device__ constexpr int CZ(int l) {
-  return (l == 5 || (l >= 7 && l <= 10)) ? 1 : (l == 6 || l >= 11) ? -1 : 0;
-}
-__host__ __device__ constexpr int WL(int l) { return l == 0 ? 16 : (l <= 6 ? 8 : 1); }
-__host__ __device__ constexpr int OPP(int l) {
-  return l == 0 ? 0 : (l <= 6 ? ((l & 1) ? l + 1 : l - 1) : 21 - l);
-}
-
-#define FLAG_SOLID 0x4000u
-#define FLAG_LINKS 0x3fffu
-
-/* Neighbour offsets along one axis for the node at coordinate q of an axis of
- * length n and stride st: off[0] for component -1 (source q+1), off[2] for
- * component +1 (source q-1); used with sign = -1 for pull (x - c) or +1
- * for push (x + c). */
-struct AxisOff {
-  long m, p;  /* offset of q-1 and q+1 with periodic wrap */
-};
-__device__ __forceinline__ AxisOff axis_off(int q, int n, long st) {
-  AxisOff o;
-  o.m = (q > 0) ? -st : (long)(n - 1) * st;
-  o.p = (q < n - 1) ? st : -(long)(n - 1) * st;
-  return o;
-}
-/* offset of x + s*c_l (s = +1 push, -1 pull) */
-template <int L, int S>
-__device__ __forceinline__ long nb_off(const AxisOff &oi, const AxisOff &oj, const AxisOff &ok) {
-  constexpr int cx = S * CX(L), cy = S * CY(L), cz = S * CZ(L);
-  long o = 0;
-  if (cx > 0) o += oi.p; else if (cx < 0) o += oi.m;
-  if (cy > 0) o += oj.p; else if (cy < 0) o += oj.m;
-  if (cz > 0) o += ok.p; else if (cz < 0) o += ok.m;
-  return o;
-}
-
-struct Geom {
-  int esi, esj, esk, plane;
-  long N;
-  int xcd_map;   /* step kernel: give each XCD a contiguous range of rows */
-  int nt_store;  /* step kernel: non-temporal population stores */
-};
-
-/* Block coordinates of the step kernel.  Workgroups are dispatched round
- * robin over the 8 XCDs; with xcd_map the linear block id b is served by
- * XCD b % 8 and that XCD walks a contiguous 1/8 of the (row, plane) space,
- * so blocks that share boundary rows share an L2. */
-__device__ __forceinline__ void step_block(const Geom &G, int &bx, int &by) {
-  bx = blockIdx.x;
-  by = blockIdx.y;
-  if (G.xcd_map) {
-    const int nbx = gridDim.x, nb = gridDim.x * gridDim.y;
-    if ((nb & 7) == 0) {
-      const int lin = by * nbx + bx;
-      const int lin2 = (lin & 7) * (nb >> 3) + (lin >> 3);
-      bx = lin2 % nbx;
-      by = lin2 / nbx;
-    }
-  }
-}
-
-/* Compile-time unrolled loop over links 1..14 */
-template <int L, int END>
-struct Links {
-  template <typename F>
-  __device__ __forceinline__ static void run(F &&f) {
-    f(std::integral_constant<int, L>{});
-    Links<L + 1, END>::run(f);
-  }
-};
-template <int END>
-struct Links<END, END> {
-  template <typename F>
-  __device__ __forceinline__ static void run(F &&) {}
-};
-
-/* ---------------------------------------------------------------------- */
-/* Collision of one node (src/lbkernel.c:797-866).                        */
-
-struct PhysArgs {
-  double omega, comega, rho0;
-  int stokes, skip_solid;
-  double bfx, bfy, bfz;
-};
-
-template <typename T>
-__device__ __forceinline__ void collide_node(T v[15], T jbx, T jby, T jbz,
-                                             const PhysArgs &P) {
-  const T ax = v[1] + v[2], dx = v[1] - v[2];
-  const T ay = v[3] + v[4], dy = v[3] - v[4];
-  const T az = v[5] + v[6], dz = v[5] - v[6];
-  /* diagonal links 7..14, c = (sx,sy,sz): sums weighted by 1, sx, sy, sz,
-   * sx sy, sx sz, sy sz */
-  const T p7 = v[7] + v[14], m7 = v[7] - v[14];     /* (+,+,+) */
-  const T p8 = v[8] + v[13], m8 = v[8] - v[13];     /* (-,+,+) */
-  const T p9 = v[9] + v[12], m9 = v[9] - v[12];     /* (+,-,+) */
-  const T p10 = v[10] + v[11], m10 = v[10] - v[11]; /* (-,-,+) */
-  const T D = (p7 + p8) + (p9 + p10);
-  const T Dx = (m7 - m8) + (m9 - m10);
-  const T Dy = (m7 + m8) - (m9 + m10);
-  const T Dz = (m7 + m8) + (m9 + m10);
-  const T Dxy = (p7 - p8) - (p9 - p10);
-  const T Dxz = (p7 - p8) + (p9 - p10);
-  const T Dyz = (p7 + p8) - (p9 + p10);
-  const T a0a = v[0] + ax + ay + az;
-  const T drho = a0a + D;
-  const T jx = dx + Dx + jbx;
-  const T jy = dy + Dy + jby;
-  const T jz = dz + Dz + jbz;
-  /* traceless-shifted stress s = Pi - drho/3 I (src/lbkernel.c:821-823) */
-  const T sh = T(2.0 / 3.0) * drho - a0a;
-  T sxx = sh + ax, syy = sh + ay, szz = sh + az;
-  T sxy = Dxy, sxz = Dxz, syz = Dyz;
-  const T cw = T(P.comega);
-  if (P.stokes) {
-    sxx *= cw; syy *= cw; szz *= cw; sxy *= cw; sxz *= cw; syz *= cw;
-  } else {
-    const T o = T(P.omega) / (T(P.rho0) + drho);
-    sxx = cw * sxx + o * jx * jx;
-    syy = cw * syy + o * jy * jy;
-    szz = cw * szz + o * jz * jz;
-    sxy = cw * sxy + o * jx * jy;
-    sxz = cw * sxz + o * jx * jz;
-    syz = cw * syz + o * jy * jz;
-  }
-  /* rebuild a_l = (w_l/72)[drho + 3 c.j + 4.5 (cc:s - tr s/3)] */
-  const T r72 = drho * T(1.0 / 72.0);
-  const T r9 = T(8.0) * r72;
-  const T t3 = (sxx + syy + szz) * T(1.0 / 3.0);
-  const T third = T(1.0 / 3.0), half = T(0.5), eighth = T(0.125), d24 = T(1.0 / 24.0);
-  v[0] = T(2.0) * r9 - t3;
-  T q, g;
-  q = r9 + half * (sxx - t3); g = third * jx; v[1] = q + g; v[2] = q - g;
-  q = r9 + half * (syy - t3); g = third * jy; v[3] = q + g; v[4] = q - g;
-  q = r9 + half * (szz - t3); g = third * jz; v[5] = q + g; v[6] = q - g;
-  q = r72 + eighth * (t3 + sxy + sxz + syz); g = d24 * (jx + jy + jz);
-  v[7] = q + g; v[14] = q - g;
-  q = r72 + eighth * (t3 - sxy - sxz + syz); g = d24 * (-jx + jy + jz);
-  v[8] = q + g; v[13] = q - g;
-  q = r72 + eighth * (t3 - sxy + sxz - syz); g = d24 * (jx - jy + jz);
-  v[9] = q + g; v[12] = q - g;
-  q = r72 + eighth * (t3 + sxy - sxz - syz); g = d24 * (-jx - jy + jz);
-  v[10] = q + g; v[11] = q - g;
-}
+#include "slb_kern.h"
 
 /* ---------------------------------------------------------------------- */
 /* Fused step kernel.                                                      */
@@ -412,18 +246,6 @@ k_flags(const int32_t *__restrict__ objidx, const int32_t *__restrict__ mask,
     if (((ms >> (l - 1)) & 1) || ((mx >> (OPP(l) - 1)) & 1)) f |= 1u << (l - 1);
   });
   flags[x] = (uint16_t)f;
-}
-
-/* Wall velocity of solid nodes from the object table, src/lbobjs.c:565-589:
- * U = rho0 (v + w x r)/12, r = node - centre (no minimum image). */
-__device__ __forceinline__ void obj_velocity(const double *__restrict__ tab, int o,
-                                             int i, int j, int k, double rho0,
-                                             double &Ux, double &Uy, double &Uz) {
-  const double *p = tab + (long)o * SLBD_OBJ_STRIDE;
-  const double rx = i - p[0], ry = j - p[1], rz = k - p[2];
-  Ux = rho0 * (p[3] + p[7] * rz - p[8] * ry) / 12.0;
-  Uy = rho0 * (p[4] + p[8] * rx - p[6] * rz) / 12.0;
-  Uz = rho0 * (p[5] + p[6] * ry - p[7] * rx) / 12.0;
 }
 
 __global__ void __launch_bounds__(256)
