@@ -66,8 +66,10 @@ __global__ void k_bstep(const T *__restrict__ src, T *__restrict__ dst,
       const long x = (long)td.y + (long)q * blockDim.x + threadIdx.x;
       if (x >= P.N) continue;
       const long N = P.N;
-      const int i = (int)(x / P.plane);
-      const int rem = (int)(x - (long)i * P.plane);
+      /* N < 2^31 per problem: 32-bit division */
+      const int xi = (int)x;
+      const int i = xi / P.plane;
+      const int rem = xi - i * P.plane;
       const int j = rem / P.esk, k = rem - j * P.esk;
       const AxisOff oi = axis_off(i, P.esi, P.plane);
       const AxisOff oj = axis_off(j, P.esj, P.esk);
@@ -95,9 +97,9 @@ __global__ void k_bstep(const T *__restrict__ src, T *__restrict__ dst,
               int si = i, sj = j, sk = k;
               if (!solid) {
                 sn = xs;
-                si = (i - CX(l) + P.esi) % P.esi;
-                sj = (j - CY(l) + P.esj) % P.esj;
-                sk = (k - CZ(l) + P.esk) % P.esk;
+                si = i - CX(l); si += (si < 0) ? P.esi : 0; si -= (si >= P.esi) ? P.esi : 0;
+                sj = j - CY(l); sj += (sj < 0) ? P.esj : 0; sj -= (sj >= P.esj) ? P.esj : 0;
+                sk = k - CZ(l); sk += (sk < 0) ? P.esk : 0; sk -= (sk >= P.esk) ? P.esk : 0;
               }
               double Ux, Uy, Uz;
               obj_velocity(tab + (long)P.tab_off * SLBD_OBJ_STRIDE, oid[sn], si, sj, sk, P.rho0,

@@ -123,6 +123,17 @@ size_t slbd_bytes(const slbd_lattice *L);
  * times; mean milliseconds per copy (HBM roofline and counter calibration). */
 double slbd_copy_test(size_t bytes, int reps);
 
+/* ---- warm start (features, targets, injection) ---- */
+long slbd_nodes(const slbd_lattice *L);
+void slbd_dims(const slbd_lattice *L, int *d);
+/* channel 0 solid flag, channel 1 signed distance * scale (periodic jump
+ * flooding); C-channel BF16 device tensor */
+void slbd_features(slbd_lattice *L, uint16_t *X, int C, float scale);
+/* channels 0..2: (j - b/2)/rho * scale on fluid nodes, 0 on solid */
+void slbd_targets(slbd_lattice *L, const slbd_params *P, float scale, uint16_t *Y, int C);
+/* equilibrium populations at rho0 and velocity U[n][0..2]*scale (fluid) */
+void slbd_inject(slbd_lattice *L, const float *U, int CU, double scale, double rho0, int stokes);
+
 /* ---- batched independent lattices (slb_batch.hip) ---- */
 typedef struct slbd_batch slbd_batch;
 slbd_batch *slbd_batch_create(int prec);

@@ -1441,3 +1441,191 @@ extern "C" double slbd_copy_test(size_t bytes, int reps) {
   HIPCHECK(hipFree(b));
   return (double)ms / reps;
 }
+
+/* ---------------------------------------------------------------------- */
+/* Warm start: network features and targets from a lattice, and equilibrium */
+/* populations from a predicted velocity field.                             */
+
+__device__ __forceinline__ void node_ijk(const Geom &G, long n, int &i, int &j, int &k) {
+  i = (int)(n / G.plane);
+  const int r = (int)(n - (long)i * G.plane);
+  j = r / G.esk;
+  k = r - j * G.esk;
+}
+
+/* periodic minimum-image squared distance between two nodes */
+__device__ __forceinline__ int pdist2(const Geom &G, long a, long b) {
+  int ai, aj, ak, bi, bj, bk;
+  node_ijk(G, a, ai, aj, ak);
+  node_ijk(G, b, bi, bj, bk);
+  int di = abs(ai - bi), dj = abs(aj - bj), dk = abs(ak - bk);
+  di = min(di, G.esi - di); dj = min(dj, G.esj - dj); dk = min(dk, G.esk - dk);
+  return di * di + dj * dj + dk * dk;
+}
+
+__global__ void k_jfa_init(const uint16_t *__restrict__ flags, int seed_solid,
+                           int *__restrict__ nearest, Geom G) {
+  const long n = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= G.N) return;
+  const int solid = (flags[n] & FLAG_SOLID) ? 1 : 0;
+  nearest[n] = (solid == seed_solid) ? (int)n : -1;
+}
+
+/* one jump-flooding pass with stride s (periodic): keep the closest seed
+ * among the 27 nodes at offsets s*(dz,dy,dx) */
+__global__ void k_jfa_pass(const int *__restrict__ in, int *__restrict__ out, int s, Geom G) {
+  const long n = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= G.N) return;
+  int i, j, k;
+  node_ijk(G, n, i, j, k);
+  int best = in[n];
+  int bd = best >= 0 ? pdist2(G, n, best) : 0x7fffffff;
+  for (int dz = -1; dz <= 1; dz++)
+    for (int dy = -1; dy <= 1; dy++)
+      for (int dx = -1; dx <= 1; dx++) {
+        int ii = ((i + dz * s) % G.esi + G.esi) % G.esi;
+        int jj = ((j + dy * s) % G.esj + G.esj) % G.esj;
+        int kk = ((k + dx * s) % G.esk + G.esk) % G.esk;
+        const int c = in[((long)ii * G.esj + jj) * G.esk + kk];
+        if (c < 0) continue;
+        const int d = pdist2(G, n, c);
+        if (d < bd) { bd = d; best = c; }
+      }
+  out[n] = best;
+}
+
+/* channels: 0 solid (1/0), 1 signed distance * scale (+ to the nearest
+ * solid node on fluid nodes, - to the nearest fluid node on solid nodes) */
+__global__ void k_features(const uint16_t *__restrict__ flags, const int *__restrict__ ns,
+                           const int *__restrict__ nf, uint16_t *__restrict__ X, int C, float scale,
+                           Geom G) {
+  const long n = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= G.N) return;
+  const bool solid = (flags[n] & FLAG_SOLID) != 0;
+  float d;
+  if (solid) d = (nf[n] >= 0) ? -sqrtf((float)pdist2(G, n, nf[n])) : -64.f;
+  else d = (ns[n] >= 0) ? sqrtf((float)pdist2(G, n, ns[n])) : 64.f;
+  for (int c = 0; c < C; c++) {
+    float v = 0.f;
+    if (c == 0) v = solid ? 1.f : 0.f;
+    else if (c == 1) v = d * scale;
+    const uint32_t u = __float_as_uint(v) + 0x7fffu + ((__float_as_uint(v) >> 16) & 1u);
+    X[n * C + c] = (uint16_t)(u >> 16);
+  }
+}
+
+/* target: corrected post-collide velocity (j - b/2)/rho * scale on fluid
+ * nodes, 0 on solid nodes, channels 0..2 of a C-channel BF16 tensor */
+template <typename T>
+__global__ void k_targets(const T *__restrict__ a, const uint16_t *__restrict__ flags, double rho0,
+                          double bx, double by, double bz, float scale, uint16_t *__restrict__ Y,
+                          int C, Geom G) {
+  const long n = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= G.N) return;
+  float u[3] = {0.f, 0.f, 0.f};
+  if (!(flags[n] & FLAG_SOLID)) {
+    double d, jx, jy, jz;
+    node_rho_j<T>(a, n, G.N, d, jx, jy, jz);
+    const double rho = rho0 + d;
+    u[0] = (float)((jx - 0.5 * bx) / rho * scale);
+    u[1] = (float)((jy - 0.5 * by) / rho * scale);
+    u[2] = (float)((jz - 0.5 * bz) / rho * scale);
+  }
+  for (int c = 0; c < C; c++) {
+    const float v = c < 3 ? u[c] : 0.f;
+    const uint32_t w = __float_as_uint(v) + 0x7fffu + ((__float_as_uint(v) >> 16) & 1u);
+    Y[n * C + c] = (uint16_t)(w >> 16);
+  }
+}
+
+/* equilibrium populations at density rho0 and velocity u = U[n][0..2]*scale
+ * (fluid nodes; u = 0 on solid nodes): drho = 0, j = rho0 u, stress 0
+ * (Stokes) or u j (the rebuild of lbloadvel, src/lbstore.c:346-399) */
+template <typename T>
+__global__ void k_inject(T *__restrict__ a, const uint16_t *__restrict__ flags,
+                         const float *__restrict__ U, int CU, double scale, double rho0, int stokes,
+                         Geom G) {
+  const long n = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= G.N) return;
+  double u[3] = {0.0, 0.0, 0.0};
+  if (!(flags[n] & FLAG_SOLID))
+    for (int c = 0; c < 3; c++) u[c] = (double)U[n * CU + c] * scale;
+  const double jx = rho0 * u[0], jy = rho0 * u[1], jz = rho0 * u[2];
+  double sxx = 0, syy = 0, szz = 0, sxy = 0, sxz = 0, syz = 0;
+  if (!stokes) {
+    sxx = u[0] * jx; syy = u[1] * jy; szz = u[2] * jz;
+    sxy = u[0] * jy; sxz = u[0] * jz; syz = u[1] * jz;
+  }
+  const double t3 = (sxx + syy + szz) / 3.0;
+  double v[15];
+  v[0] = -t3;
+  double q, g;
+  q = 0.5 * (sxx - t3); g = jx / 3.0; v[1] = q + g; v[2] = q - g;
+  q = 0.5 * (syy - t3); g = jy / 3.0; v[3] = q + g; v[4] = q - g;
+  q = 0.5 * (szz - t3); g = jz / 3.0; v[5] = q + g; v[6] = q - g;
+  q = 0.125 * (t3 + sxy + sxz + syz); g = (jx + jy + jz) / 24.0; v[7] = q + g; v[14] = q - g;
+  q = 0.125 * (t3 - sxy - sxz + syz); g = (-jx + jy + jz) / 24.0; v[8] = q + g; v[13] = q - g;
+  q = 0.125 * (t3 - sxy + sxz - syz); g = (jx - jy + jz) / 24.0; v[9] = q + g; v[12] = q - g;
+  q = 0.125 * (t3 + sxy - sxz - syz); g = (-jx - jy + jz) / 24.0; v[10] = q + g; v[11] = q - g;
+#pragma unroll
+  for (int l = 0; l < 15; l++) a[(long)l * G.N + n] = (T)v[l];
+}
+
+extern "C" long slbd_nodes(const slbd_lattice *L) { return L->N; }
+extern "C" void slbd_dims(const slbd_lattice *L, int *d) { d[0] = L->esi; d[1] = L->esj; d[2] = L->esk; }
+
+/* features of the lattice's solid map into a C-channel BF16 device tensor */
+extern "C" void slbd_features(slbd_lattice *L, uint16_t *X, int C, float scale) {
+  int *buf[3];
+  for (int b = 0; b < 3; b++) HIPCHECK(hipMalloc(&buf[b], sizeof(int) * L->N));
+  int maxd = L->esi > L->esj ? L->esi : L->esj;
+  maxd = maxd > L->esk ? maxd : L->esk;
+  int s0 = 1;
+  while (2 * s0 < maxd) s0 *= 2;
+  /* phase 0 seeds on solid nodes (distance to solid), phase 1 on fluid */
+  int *ping = buf[0], *pong = buf[1], *res[2];
+  for (int phase = 0; phase < 2; phase++) {
+    if (phase == 1) pong = buf[2];
+    hipLaunchKernelGGL(k_jfa_init, lin_grid(L->N), dim3(256), 0, L->stream, L->flags,
+                       phase == 0 ? 1 : 0, ping, L->G);
+    LAUNCHCHECK();
+    for (int s = s0; s >= 1; s /= 2) {
+      hipLaunchKernelGGL(k_jfa_pass, lin_grid(L->N), dim3(256), 0, L->stream, ping, pong, s, L->G);
+      LAUNCHCHECK();
+      int *t = ping; ping = pong; pong = t;
+    }
+    /* one more stride-1 pass (JFA+1) corrects most of the residual errors */
+    hipLaunchKernelGGL(k_jfa_pass, lin_grid(L->N), dim3(256), 0, L->stream, ping, pong, 1, L->G);
+    LAUNCHCHECK();
+    res[phase] = pong;   /* ping is free for the next phase */
+  }
+  hipLaunchKernelGGL(k_features, lin_grid(L->N), dim3(256), 0, L->stream, L->flags, res[0], res[1],
+                     X, C, scale, L->G);
+  LAUNCHCHECK();
+  HIPCHECK(hipStreamSynchronize(L->stream));
+  for (int b = 0; b < 3; b++) HIPCHECK(hipFree(buf[b]));
+}
+
+extern "C" void slbd_targets(slbd_lattice *L, const slbd_params *P, float scale, uint16_t *Y, int C) {
+  const double bx = P->bf[0], by = P->bf[1], bz = P->bf[2];
+  if (L->prec == 32)
+    hipLaunchKernelGGL(k_targets<float>, lin_grid(L->N), dim3(256), 0, L->stream,
+                       (const float *)L->pops[L->cur], L->flags, P->rho0, bx, by, bz, scale, Y, C, L->G);
+  else
+    hipLaunchKernelGGL(k_targets<double>, lin_grid(L->N), dim3(256), 0, L->stream,
+                       (const double *)L->pops[L->cur], L->flags, P->rho0, bx, by, bz, scale, Y, C, L->G);
+  LAUNCHCHECK();
+  HIPCHECK(hipStreamSynchronize(L->stream));
+}
+
+extern "C" void slbd_inject(slbd_lattice *L, const float *U, int CU, double scale, double rho0,
+                            int stokes) {
+  if (L->prec == 32)
+    hipLaunchKernelGGL(k_inject<float>, lin_grid(L->N), dim3(256), 0, L->stream,
+                       (float *)L->pops[L->cur], L->flags, U, CU, scale, rho0, stokes, L->G);
+  else
+    hipLaunchKernelGGL(k_inject<double>, lin_grid(L->N), dim3(256), 0, L->stream,
+                       (double *)L->pops[L->cur], L->flags, U, CU, scale, rho0, stokes, L->G);
+  LAUNCHCHECK();
+  HIPCHECK(hipStreamSynchronize(L->stream));
+}

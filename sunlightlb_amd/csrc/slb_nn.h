@@ -1,0 +1,45 @@
+/*
+ * slb_nn.h -- internal C ABI of the warm-start network kernels (slb_nn.hip).
+ * Tensors are device pointers, channels-last BF16 (uint16_t) with channel
+ * counts that are multiples of 8; weights W[co][Kp], Kp = slbn_kpad(cin),
+ * rows padded with zeros to a multiple of 16.
+ */
+#ifndef SLB_NN_H
+#define SLB_NN_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+int  slbn_kpad(int cin);
+void slbn_set_stream(void *stream);
+/* Y = act(bias + conv3x3x3(X, W)), periodic padding; transposed != 0 uses
+ * the mirrored taps (gradient of the input with W = slbn_wtrans(W)).  Y
+ * (BF16) and/or Yf (FP32) may be NULL. */
+void slbn_conv(const uint16_t *X, int cin, const uint16_t *W, const float *bias, uint16_t *Y,
+               float *Yf, int cout, int relu, int transposed, int D, int H, int Wd);
+/* dW[co][t*cin+ci] += sum_n dY[n][co] X[n+off(t)][ci]; channels 16..64 step 16 */
+void slbn_conv_wgrad(const uint16_t *X, int cin, const uint16_t *dY, int cout, float *dW, int D,
+                     int H, int Wd);
+/* the same for any channel counts that are multiples of 8 (scalar FP32) */
+void slbn_conv_wgrad_small(const uint16_t *X, int cin, const uint16_t *dY, int cout, float *dW,
+                           int D, int H, int Wd);
+void slbn_wtrans(const uint16_t *W, int cout, int cin, uint16_t *WT);
+void slbn_relu_bwd(const uint16_t *Y, const uint16_t *dY, uint16_t *dP, long n);
+void slbn_bias_grad(const uint16_t *dP, int C, int N, float *db);
+void slbn_pool(const uint16_t *X, uint16_t *Y, int C, int D, int H, int Wd);
+void slbn_pool_bwd(const uint16_t *dY, uint16_t *dX, int C, int D, int H, int Wd);
+void slbn_up_add(const uint16_t *Xc, const uint16_t *S, uint16_t *Y, int C, int D, int H, int Wd);
+void slbn_up_bwd(const uint16_t *dY, uint16_t *dXc, int C, int D, int H, int Wd);
+void slbn_add_into(uint16_t *A, const uint16_t *B, long n);
+void slbn_mse(const float *Yf, int C, const uint16_t *T, int Ct, int Cv, int N, float scale,
+              uint16_t *dY, float *loss);
+void slbn_adam(float *w, uint16_t *wb, const float *g, float *m, float *v, long n, float lr,
+               float b1, float b2, float eps, int step);
+void slbn_f2bf(const float *a, uint16_t *b, long n);
+void slbn_bf2f(const uint16_t *a, float *b, long n);
+/* expand a packed sample (Cs channels) into a C-channel tensor, zeros beyond Cs */
+void slbn_expand(const uint16_t *S, int Cs, uint16_t *X, int C, long N);
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,518 @@
+/*
+ * slb_nn.hip -- gfx950 kernels of the learned warm start: a 3D convolutional
+ * encoder-decoder over the lattice, trained and evaluated on the GPU.
+ *
+ * Tensors are channels-last BF16, X[n][c] with n the lattice node index
+ * (esk*(esj*i+j)+k) and C a multiple of 8, so the 8 channels one MFMA lane
+ * needs are one 16-byte load.  All convolutions are 3x3x3 with periodic
+ * padding (the lattice is periodic) and are implicit GEMMs on the BF16 MFMA
+ * v_mfma_f32_16x16x32_bf16 with FP32 accumulation:
+ *
+ *   forward      M = Cout, N = 16 consecutive nodes, K = 27*Cin (k = tap*Cin + ci)
+ *   grad input   M = Cin,  N = 16 consecutive nodes, K = 27*Cout (transposed taps)
+ *   grad weight  M = Cout, N = Cin (one tap), K = nodes, staged through LDS
+ *
+ * MFMA operand layout (checked on gfx950): lane l holds A[l%16][(l/16)*8+e]
+ * and B[(l/16)*8+e][l%16] (e = 0..7) and receives D[(l/16)*4+r][l%16]
+ * (r = 0..3).  Weights are W[co][Kp] with Kp = 27*Cin rounded up to 32 and
+ * zero-filled.
+ */
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include "slb_nn.h"
+
+#define HIPCHECK(call)                                                        \
+  do {                                                                        \
+    hipError_t e_ = (call);                                                   \
+    if (e_ != hipSuccess) {                                                   \
+      fprintf(stderr, "sunlightlb: HIP error '%s' (%d) at %s:%d in %s\n",     \
+              hipGetErrorString(e_), (int)e_, __FILE__, __LINE__, #call);     \
+      fflush(stderr);                                                         \
+      exit(EXIT_FAILURE);                                                     \
+    }                                                                         \
+  } while (0)
+#define LAUNCHCHECK() HIPCHECK(hipGetLastError())
+
+typedef __attribute__((ext_vector_type(8))) __bf16 bf16x8;
+typedef __attribute__((ext_vector_type(4))) float f32x4;
+
+__device__ __forceinline__ float bf2f(uint16_t h) {
+  return __uint_as_float(((uint32_t)h) << 16);
+}
+/* round to nearest even */
+__device__ __forceinline__ uint16_t f2bf(float f) {
+  uint32_t u = __float_as_uint(f);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40);  /* NaN */
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+
+struct NGeom {
+  int D, H, Wd;     /* i, j, k extents */
+  int plane;        /* H*Wd */
+  int N;            /* nodes */
+};
+
+/* node n shifted by tap t (dz,dy,dx) = (t/9-1, (t/3)%3-1, t%3-1), times sgn,
+ * periodic */
+__device__ __forceinline__ int tap_node(const NGeom &G, int z, int y, int x, int t, int sgn) {
+  int dz = (t / 9 - 1) * sgn, dy = ((t / 3) % 3 - 1) * sgn, dx = (t % 3 - 1) * sgn;
+  int zz = z + dz, yy = y + dy, xx = x + dx;
+  zz += (zz < 0) ? G.D : 0; zz -= (zz >= G.D) ? G.D : 0;
+  yy += (yy < 0) ? G.H : 0; yy -= (yy >= G.H) ? G.H : 0;
+  xx += (xx < 0) ? G.Wd : 0; xx -= (xx >= G.Wd) ? G.Wd : 0;
+  return (zz * G.H + yy) * G.Wd + xx;
+}
+
+/* ---------------------------------------------------------------------- */
+/* Forward / grad-input convolution (same kernel, tap direction sgn).       */
+/* Y[n][co] = act( bias[co] + sum_k W[co][k] * X[n + sgn*off(tap(k))][ci(k)] ) */
+/* Block: 256 threads = 4 waves; wave w handles nodes n0 + 16w .. +15 and all  */
+/* output channels (MT = Cout/16 tiles).                                     */
+
+template <int MT>
+__global__ void __launch_bounds__(256)
+k_conv(const uint16_t *__restrict__ X, int Cin, const uint16_t *__restrict__ W, int Kp,
+       const float *__restrict__ bias, uint16_t *__restrict__ Y, float *__restrict__ Yf,
+       int Cout, int relu, int sgn, NGeom G) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int n0 = (blockIdx.x * 4 + wv) * 16;
+  if (n0 >= G.N) return;
+  const int n = n0 + (lane & 15);
+  const int grp = lane >> 4;
+  const bool valid = n < G.N;
+  const int nn = valid ? n : n0;
+  const int z = nn / G.plane, rem = nn - z * G.plane, y = rem / G.Wd, x = rem - y * G.Wd;
+  const int K = 27 * Cin;
+  f32x4 acc[MT];
+#pragma unroll
+  for (int m = 0; m < MT; m++) acc[m] = (f32x4){0.f, 0.f, 0.f, 0.f};
+  for (int k0 = 0; k0 < Kp; k0 += 32) {
+    const int kk = k0 + 8 * grp;
+    bf16x8 b;
+    if (kk < K) {
+      const int t = kk / Cin, ci = kk - t * Cin;
+      const int src = tap_node(G, z, y, x, t, sgn);
+      b = *(const bf16x8 *)(X + (long)src * Cin + ci);
+    } else {
+      b = (bf16x8){0, 0, 0, 0, 0, 0, 0, 0};
+    }
+#pragma unroll
+    for (int m = 0; m < MT; m++) {
+      const int co = 16 * m + (lane & 15);
+      const bf16x8 a = *(const bf16x8 *)(W + (long)co * Kp + kk);
+      acc[m] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[m], 0, 0, 0);
+    }
+  }
+  if (!valid) return;
+  /* lane holds D[co = 16m + 4*grp + r][node n] */
+#pragma unroll
+  for (int m = 0; m < MT; m++) {
+    const int cb = 16 * m + 4 * grp;
+    if (cb >= Cout) continue;
+    float v[4];
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      v[r] = acc[m][r] + (bias ? bias[cb + r] : 0.f);
+      if (relu) v[r] = v[r] > 0.f ? v[r] : 0.f;
+    }
+    if (Y) {
+      uint2 pk;
+      pk.x = (uint32_t)f2bf(v[0]) | ((uint32_t)f2bf(v[1]) << 16);
+      pk.y = (uint32_t)f2bf(v[2]) | ((uint32_t)f2bf(v[3]) << 16);
+      *(uint2 *)(Y + (long)n * Cout + cb) = pk;
+    }
+    if (Yf) {
+#pragma unroll
+      for (int r = 0; r < 4; r++) Yf[(long)n * Cout + cb + r] = v[r];
+    }
+  }
+}
+
+/* ---------------------------------------------------------------------- */
+/* Weight gradient for one tap t: dW[co][t*Cin+ci] += sum_n dY[n][co] *     */
+/* X[n + off(t)][ci].  Block: 256 threads; wave w computes co tile w (if    */
+/* w < Cout/16) against all Cin/16 ci tiles; K = nodes in chunks of 32,      */
+/* staged through LDS transposed ([c][node]).                               */
+
+#define WG_CHUNK 32
+template <int NT>
+__global__ void __launch_bounds__(256)
+k_conv_wgrad(const uint16_t *__restrict__ X, int Cin, const uint16_t *__restrict__ dY, int Cout,
+             float *__restrict__ dW, int Kp, int t, int nodes_per_block, NGeom G) {
+  __shared__ uint16_t sD[64 * WG_CHUNK];   /* [co][node] */
+  __shared__ uint16_t sX[64 * WG_CHUNK];   /* [ci][node] */
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int grp = lane >> 4;
+  const int nbeg = blockIdx.x * nodes_per_block;
+  const int nend = min(G.N, nbeg + nodes_per_block);
+  f32x4 acc[NT];
+#pragma unroll
+  for (int q = 0; q < NT; q++) acc[q] = (f32x4){0.f, 0.f, 0.f, 0.f};
+  const bool active = (wv * 16) < Cout;
+  for (int c0 = nbeg; c0 < nend; c0 += WG_CHUNK) {
+    __syncthreads();
+    /* stage dY[c0..c0+31][0..Cout) and X[shift(c0..)][0..Cin) transposed */
+    for (int e = threadIdx.x; e < WG_CHUNK * (Cout / 8); e += blockDim.x) {
+      const int p = e / (Cout / 8), cg = (e - p * (Cout / 8)) * 8;
+      const int n = c0 + p;
+      uint16_t v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+      if (n < nend) *(uint4 *)v = *(const uint4 *)(dY + (long)n * Cout + cg);
+#pragma unroll
+      for (int q = 0; q < 8; q++) sD[(cg + q) * WG_CHUNK + p] = v[q];
+    }
+    for (int e = threadIdx.x; e < WG_CHUNK * (Cin / 8); e += blockDim.x) {
+      const int p = e / (Cin / 8), cg = (e - p * (Cin / 8)) * 8;
+      const int n = c0 + p;
+      uint16_t v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+      if (n < nend) {
+        const int z = n / G.plane, rem = n - z * G.plane, y = rem / G.Wd, x = rem - y * G.Wd;
+        const int src = tap_node(G, z, y, x, t, 1);
+        *(uint4 *)v = *(const uint4 *)(X + (long)src * Cin + cg);
+      }
+#pragma unroll
+      for (int q = 0; q < 8; q++) sX[(cg + q) * WG_CHUNK + p] = v[q];
+    }
+    __syncthreads();
+    if (active) {
+      const int co = wv * 16 + (lane & 15);
+      const bf16x8 a = *(const bf16x8 *)(sD + co * WG_CHUNK + 8 * grp);
+#pragma unroll
+      for (int q = 0; q < NT; q++) {
+        const int ci = q * 16 + (lane & 15);
+        const bf16x8 b = *(const bf16x8 *)(sX + ci * WG_CHUNK + 8 * grp);
+        acc[q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[q], 0, 0, 0);
+      }
+    }
+  }
+  if (!active) return;
+  /* lane holds D[co = wv*16 + 4*grp + r][ci = q*16 + lane%16] */
+#pragma unroll
+  for (int q = 0; q < NT; q++) {
+    const int ci = q * 16 + (lane & 15);
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int co = wv * 16 + 4 * grp + r;
+      atomicAdd(dW + (long)co * Kp + t * Cin + ci, acc[q][r]);
+    }
+  }
+}
+
+/* ---------------------------------------------------------------------- */
+/* Elementwise and reduction helpers.                                        */
+
+/* WT[ci][t*Cout + co] = W[co][t*Cin + ci]  (taps of the transposed conv) */
+__global__ void k_wtrans(const uint16_t *__restrict__ W, int Cout, int Cin, int Kp,
+                         uint16_t *__restrict__ WT, int KpT) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= Cin * KpT) return;
+  const int ci = e / KpT, k = e - ci * KpT;
+  uint16_t v = 0;
+  if (k < 27 * Cout) {
+    const int t = k / Cout, co = k - t * Cout;
+    v = W[(long)co * Kp + t * Cin + ci];
+  }
+  WT[e] = v;
+}
+
+/* dPre = dY * (Y > 0) (ReLU derivative from the stored output) */
+__global__ void k_relu_bwd(const uint16_t *__restrict__ Y, const uint16_t *__restrict__ dY,
+                           uint16_t *__restrict__ dP, long n) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  dP[e] = (bf2f(Y[e]) > 0.f) ? dY[e] : (uint16_t)0;
+}
+
+/* db[c] += sum_n dP[n][c] */
+__global__ void __launch_bounds__(256)
+k_bias_grad(const uint16_t *__restrict__ dP, int C, int N, float *__restrict__ db) {
+  const int c = threadIdx.x & 63;
+  const int rowgrp = threadIdx.x >> 6;
+  if (c >= C) return;
+  float s = 0.f;
+  for (int n = blockIdx.x * 4 + rowgrp; n < N; n += gridDim.x * 4) s += bf2f(dP[(long)n * C + c]);
+  atomicAdd(db + c, s);
+}
+
+/* 2x2x2 average pooling (channels-last) and its gradient */
+__global__ void k_pool(const uint16_t *__restrict__ X, uint16_t *__restrict__ Y, int C, NGeom Gi,
+                       NGeom Go) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long)Go.N * C) return;
+  const int n = (int)(e / C), c = (int)(e - (long)n * C);
+  const int z = n / Go.plane, rem = n - z * Go.plane, y = rem / Go.Wd, x = rem - y * Go.Wd;
+  float s = 0.f;
+  for (int a = 0; a < 2; a++)
+    for (int b = 0; b < 2; b++)
+      for (int d = 0; d < 2; d++)
+        s += bf2f(X[(long)(((2 * z + a) * Gi.H + 2 * y + b) * Gi.Wd + 2 * x + d) * C + c]);
+  Y[e] = f2bf(0.125f * s);
+}
+__global__ void k_pool_bwd(const uint16_t *__restrict__ dY, uint16_t *__restrict__ dX, int C,
+                           NGeom Gi, NGeom Go) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long)Gi.N * C) return;
+  const int n = (int)(e / C), c = (int)(e - (long)n * C);
+  const int z = n / Gi.plane, rem = n - z * Gi.plane, y = rem / Gi.Wd, x = rem - y * Gi.Wd;
+  const int m = ((z / 2) * Go.H + y / 2) * Go.Wd + x / 2;
+  dX[e] = f2bf(0.125f * bf2f(dY[(long)m * C + c]));
+}
+
+/* nearest x2 upsampling added to a skip tensor: Y = up(Xc) + S; the
+ * gradient of Xc is the sum over the 8 children */
+__global__ void k_up_add(const uint16_t *__restrict__ Xc, const uint16_t *__restrict__ S,
+                         uint16_t *__restrict__ Y, int C, NGeom Gf, NGeom Gc) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long)Gf.N * C) return;
+  const int n = (int)(e / C), c = (int)(e - (long)n * C);
+  const int z = n / Gf.plane, rem = n - z * Gf.plane, y = rem / Gf.Wd, x = rem - y * Gf.Wd;
+  const int m = ((z / 2) * Gc.H + y / 2) * Gc.Wd + x / 2;
+  Y[e] = f2bf(bf2f(Xc[(long)m * C + c]) + (S ? bf2f(S[e]) : 0.f));
+}
+__global__ void k_up_bwd(const uint16_t *__restrict__ dY, uint16_t *__restrict__ dXc, int C,
+                         NGeom Gf, NGeom Gc) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long)Gc.N * C) return;
+  const int n = (int)(e / C), c = (int)(e - (long)n * C);
+  const int z = n / Gc.plane, rem = n - z * Gc.plane, y = rem / Gc.Wd, x = rem - y * Gc.Wd;
+  float s = 0.f;
+  for (int a = 0; a < 2; a++)
+    for (int b = 0; b < 2; b++)
+      for (int d = 0; d < 2; d++)
+        s += bf2f(dY[(long)(((2 * z + a) * Gf.H + 2 * y + b) * Gf.Wd + 2 * x + d) * C + c]);
+  dXc[e] = f2bf(s);
+}
+
+/* dA += dB elementwise (skip gradients), BF16 storage, FP32 add */
+__global__ void k_add_into(uint16_t *__restrict__ A, const uint16_t *__restrict__ B, long n) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  A[e] = f2bf(bf2f(A[e]) + bf2f(B[e]));
+}
+
+/* Weighted mean-square error of an FP32 prediction against a BF16 target
+ * over the first Cv channels, gradient into dY (BF16, C channels): the sum
+ * goes to loss[0] via atomics. */
+__global__ void __launch_bounds__(256)
+k_mse(const float *__restrict__ Yf, int C, const uint16_t *__restrict__ T, int Ct, int Cv,
+      int N, float scale, uint16_t *__restrict__ dY, float *__restrict__ loss) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  float s = 0.f;
+  if (n < N) {
+    for (int c = 0; c < C; c++) {
+      float g = 0.f;
+      if (c < Cv) {
+        const float d = Yf[(long)n * C + c] - bf2f(T[(long)n * Ct + c]);
+        s += d * d;
+        g = 2.f * d * scale;
+      }
+      dY[(long)n * C + c] = f2bf(g);
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  if ((threadIdx.x & 63) == 0) atomicAdd(loss, s);
+}
+
+/* Adam on FP32 master weights, refreshing the BF16 copy (zero-padded K
+ * columns stay zero because their gradient is zero). */
+__global__ void k_adam(float *__restrict__ w, uint16_t *__restrict__ wb, const float *__restrict__ g,
+                       float *__restrict__ m, float *__restrict__ v, long n, float lr, float b1,
+                       float b2, float eps, float c1, float c2) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  const float gg = g[e];
+  const float mm = b1 * m[e] + (1.f - b1) * gg;
+  const float vv = b2 * v[e] + (1.f - b2) * gg * gg;
+  m[e] = mm;
+  v[e] = vv;
+  const float ww = w[e] - lr * (mm / c1) / (sqrtf(vv / c2) + eps);
+  w[e] = ww;
+  if (wb) wb[e] = f2bf(ww);
+}
+
+__global__ void k_f2bf(const float *__restrict__ a, uint16_t *__restrict__ b, long n) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < n) b[e] = f2bf(a[e]);
+}
+__global__ void k_bf2f(const uint16_t *__restrict__ a, float *__restrict__ b, long n) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < n) b[e] = bf2f(a[e]);
+}
+
+/* ---------------------------------------------------------------------- */
+/* Host ABI.                                                                */
+
+static NGeom ngeom(int D, int H, int Wd) {
+  NGeom g;
+  g.D = D; g.H = H; g.Wd = Wd; g.plane = H * Wd; g.N = D * H * Wd;
+  return g;
+}
+
+static hipStream_t nn_stream = 0;
+extern "C" void slbn_set_stream(void *s) { nn_stream = (hipStream_t)s; }
+
+static unsigned blocks_for(long n, int b) { return (unsigned)((n + b - 1) / b); }
+
+extern "C" int slbn_kpad(int cin) { return ((27 * cin + 31) / 32) * 32; }
+
+extern "C" void slbn_conv(const uint16_t *X, int cin, const uint16_t *W, const float *bias,
+                          uint16_t *Y, float *Yf, int cout, int relu, int transposed, int D, int H,
+                          int Wd) {
+  const NGeom G = ngeom(D, H, Wd);
+  const int Kp = slbn_kpad(cin);
+  const unsigned g = blocks_for(G.N, 64);
+  const int sgn = transposed ? -1 : 1;
+#define SLBN_CONV(MT)                                                                      \
+  hipLaunchKernelGGL(k_conv<MT>, dim3(g), dim3(256), 0, nn_stream, X, cin, W, Kp, bias, Y, Yf, \
+                     cout, relu, sgn, G)
+  const int mt = (cout + 15) / 16;
+  if (mt == 1) SLBN_CONV(1);
+  else if (mt == 2) SLBN_CONV(2);
+  else if (mt == 3) SLBN_CONV(3);
+  else if (mt == 4) SLBN_CONV(4);
+  else { fprintf(stderr, "slbn_conv: at most 64 output channels\n"); exit(EXIT_FAILURE); }
+#undef SLBN_CONV
+  LAUNCHCHECK();
+}
+
+extern "C" void slbn_conv_wgrad(const uint16_t *X, int cin, const uint16_t *dY, int cout,
+                                float *dW, int D, int H, int Wd) {
+  const NGeom G = ngeom(D, H, Wd);
+  const int Kp = slbn_kpad(cin);
+  if (cin > 64 || cout > 64 || (cin % 16) || (cout % 16)) {
+    fprintf(stderr, "slbn_conv_wgrad: channels must be 16..64 in steps of 16\n");
+    exit(EXIT_FAILURE);
+  }
+  /* about 256 node chunks per block keeps the register accumulation long */
+  int npb = WG_CHUNK * 64;
+  if ((long)npb * 64 < G.N) npb = (int)(((G.N + 63) / 64 + WG_CHUNK - 1) / WG_CHUNK) * WG_CHUNK;
+  const unsigned g = blocks_for(G.N, npb);
+  for (int t = 0; t < 27; t++) {
+#define SLBN_WG(NT)                                                                       \
+  hipLaunchKernelGGL(k_conv_wgrad<NT>, dim3(g), dim3(256), 0, nn_stream, X, cin, dY, cout, dW, \
+                     Kp, t, npb, G)
+    const int nt = cin / 16;
+    if (nt == 1) SLBN_WG(1);
+    else if (nt == 2) SLBN_WG(2);
+    else if (nt == 3) SLBN_WG(3);
+    else SLBN_WG(4);
+#undef SLBN_WG
+    LAUNCHCHECK();
+  }
+}
+
+extern "C" void slbn_wtrans(const uint16_t *W, int cout, int cin, uint16_t *WT) {
+  const int Kp = slbn_kpad(cin), KpT = slbn_kpad(cout);
+  hipLaunchKernelGGL(k_wtrans, dim3(blocks_for((long)cin * KpT, 256)), dim3(256), 0, nn_stream,
+                     W, cout, cin, Kp, WT, KpT);
+  LAUNCHCHECK();
+}
+
+extern "C" void slbn_relu_bwd(const uint16_t *Y, const uint16_t *dY, uint16_t *dP, long n) {
+  hipLaunchKernelGGL(k_relu_bwd, dim3(blocks_for(n, 256)), dim3(256), 0, nn_stream, Y, dY, dP, n);
+  LAUNCHCHECK();
+}
+
+extern "C" void slbn_bias_grad(const uint16_t *dP, int C, int N, float *db) {
+  hipLaunchKernelGGL(k_bias_grad, dim3(256), dim3(256), 0, nn_stream, dP, C, N, db);
+  LAUNCHCHECK();
+}
+
+extern "C" void slbn_pool(const uint16_t *X, uint16_t *Y, int C, int D, int H, int Wd) {
+  const NGeom Gi = ngeom(D, H, Wd), Go = ngeom(D / 2, H / 2, Wd / 2);
+  hipLaunchKernelGGL(k_pool, dim3(blocks_for((long)Go.N * C, 256)), dim3(256), 0, nn_stream, X, Y,
+                     C, Gi, Go);
+  LAUNCHCHECK();
+}
+extern "C" void slbn_pool_bwd(const uint16_t *dY, uint16_t *dX, int C, int D, int H, int Wd) {
+  const NGeom Gi = ngeom(D, H, Wd), Go = ngeom(D / 2, H / 2, Wd / 2);
+  hipLaunchKernelGGL(k_pool_bwd, dim3(blocks_for((long)Gi.N * C, 256)), dim3(256), 0, nn_stream,
+                     dY, dX, C, Gi, Go);
+  LAUNCHCHECK();
+}
+extern "C" void slbn_up_add(const uint16_t *Xc, const uint16_t *S, uint16_t *Y, int C, int D, int H,
+                            int Wd) {
+  const NGeom Gf = ngeom(D, H, Wd), Gc = ngeom(D / 2, H / 2, Wd / 2);
+  hipLaunchKernelGGL(k_up_add, dim3(blocks_for((long)Gf.N * C, 256)), dim3(256), 0, nn_stream, Xc,
+                     S, Y, C, Gf, Gc);
+  LAUNCHCHECK();
+}
+extern "C" void slbn_up_bwd(const uint16_t *dY, uint16_t *dXc, int C, int D, int H, int Wd) {
+  const NGeom Gf = ngeom(D, H, Wd), Gc = ngeom(D / 2, H / 2, Wd / 2);
+  hipLaunchKernelGGL(k_up_bwd, dim3(blocks_for((long)Gc.N * C, 256)), dim3(256), 0, nn_stream, dY,
+                     dXc, C, Gf, Gc);
+  LAUNCHCHECK();
+}
+extern "C" void slbn_add_into(uint16_t *A, const uint16_t *B, long n) {
+  hipLaunchKernelGGL(k_add_into, dim3(blocks_for(n, 256)), dim3(256), 0, nn_stream, A, B, n);
+  LAUNCHCHECK();
+}
+extern "C" void slbn_mse(const float *Yf, int C, const uint16_t *T, int Ct, int Cv, int N,
+                         float scale, uint16_t *dY, float *loss) {
+  hipLaunchKernelGGL(k_mse, dim3(blocks_for(N, 256)), dim3(256), 0, nn_stream, Yf, C, T, Ct, Cv, N,
+                     scale, dY, loss);
+  LAUNCHCHECK();
+}
+extern "C" void slbn_adam(float *w, uint16_t *wb, const float *g, float *m, float *v, long n,
+                          float lr, float b1, float b2, float eps, int step) {
+  const float c1 = 1.f - powf(b1, (float)step), c2 = 1.f - powf(b2, (float)step);
+  hipLaunchKernelGGL(k_adam, dim3(blocks_for(n, 256)), dim3(256), 0, nn_stream, w, wb, g, m, v, n,
+                     lr, b1, b2, eps, c1, c2);
+  LAUNCHCHECK();
+}
+extern "C" void slbn_f2bf(const float *a, uint16_t *b, long n) {
+  hipLaunchKernelGGL(k_f2bf, dim3(blocks_for(n, 256)), dim3(256), 0, nn_stream, a, b, n);
+  LAUNCHCHECK();
+}
+extern "C" void slbn_bf2f(const uint16_t *a, float *b, long n) {
+  hipLaunchKernelGGL(k_bf2f, dim3(blocks_for(n, 256)), dim3(256), 0, nn_stream, a, b, n);
+  LAUNCHCHECK();
+}
+
+/* Weight gradient for small channel counts: one thread per (co, t, ci)
+ * element and node chunk, FP32, atomics into dW. */
+__global__ void __launch_bounds__(256)
+k_conv_wgrad_small(const uint16_t *__restrict__ X, int Cin, const uint16_t *__restrict__ dY,
+                   int Cout, float *__restrict__ dW, int Kp, int chunk, NGeom G) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= Cout * 27 * Cin) return;
+  const int co = e / (27 * Cin), rem = e - co * 27 * Cin, t = rem / Cin, ci = rem - t * Cin;
+  const int nb = blockIdx.y * chunk, ne = min(G.N, nb + chunk);
+  float s = 0.f;
+  for (int n = nb; n < ne; n++) {
+    const float g = bf2f(dY[(long)n * Cout + co]);
+    if (g == 0.f) continue;
+    const int z = n / G.plane, r2 = n - z * G.plane, y = r2 / G.Wd, x = r2 - y * G.Wd;
+    s += g * bf2f(X[(long)tap_node(G, z, y, x, t, 1) * Cin + ci]);
+  }
+  if (s != 0.f) atomicAdd(dW + (long)co * Kp + t * Cin + ci, s);
+}
+
+extern "C" void slbn_conv_wgrad_small(const uint16_t *X, int cin, const uint16_t *dY, int cout,
+                                      float *dW, int D, int H, int Wd) {
+  const NGeom G = ngeom(D, H, Wd);
+  const int Kp = slbn_kpad(cin);
+  const int chunk = 4096;
+  dim3 g(blocks_for((long)cout * 27 * cin, 256), blocks_for(G.N, chunk));
+  hipLaunchKernelGGL(k_conv_wgrad_small, g, dim3(256), 0, nn_stream, X, cin, dY, cout, dW, Kp,
+                     chunk, G);
+  LAUNCHCHECK();
+}
+
+__global__ void k_expand(const uint16_t *__restrict__ S, int Cs, uint16_t *__restrict__ X, int C,
+                         long N) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= N * C) return;
+  const long n = e / C;
+  const int c = (int)(e - n * C);
+  X[e] = (c < Cs) ? S[n * Cs + c] : (uint16_t)0;
+}
+
+extern "C" void slbn_expand(const uint16_t *S, int Cs, uint16_t *X, int C, long N) {
+  hipLaunchKernelGGL(k_expand, dim3(blocks_for(N * C, 256)), dim3(256), 0, nn_stream, S, Cs, X, C, N);
+  LAUNCHCHECK();
+}
