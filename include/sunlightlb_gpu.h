@@ -10,6 +10,7 @@
 #ifndef SUNLIGHTLB_GPU_H_INC
 #define SUNLIGHTLB_GPU_H_INC
 
+#include <stdint.h>
 #include "sunlightlb.h"
 
 #ifdef __cplusplus
@@ -169,6 +170,87 @@ double slb_hbm_copy_test(size_t bytes, int reps);
 size_t slb_device_bytes(LBSys lb);
 /* GPU architecture name, e.g. "gfx950". */
 const char *slb_device_arch(void);
+
+/* ---- learned warm start (no reference counterpart) ----
+ * A 3D convolutional encoder-decoder (4 levels, 16-64 channels, 3x3x3 taps,
+ * periodic padding; BF16 weights and activations, FP32 accumulation, all on
+ * the GPU) maps the solid flag and signed distance of a lattice to the 10
+ * hydrodynamic moments of its steady Stokes state per unit body force F:
+ * m[0..2] corrected velocity u, m[3] density deviation drho, m[4..9] stress
+ * s = Pi - drho/3 I (xx, yy, zz, xy, xz, yz), zero on solid nodes.  Needs a
+ * uniform non-zero body force; lattice edges must be multiples of 8 (>= 16).
+ * Initialisations (slb_warm_set_mode):
+ *   SLB_WARM_EQUILIBRIUM  equilibrium at rho0 and the predicted velocity,
+ *                         then one collide: what lbloadvel builds;
+ *   SLB_WARM_MOMENTS      (default) the post-collide state rebuilt from all
+ *                         10 predicted moments. */
+#define SLB_WARM_EQUILIBRIUM 0
+#define SLB_WARM_MOMENTS 1
+typedef struct slb_warmnet slb_warmnet;
+/* New network with seeded He-initialised weights. */
+slb_warmnet *slb_warm_new(unsigned long long seed);
+void   slb_warm_free(slb_warmnet *net);
+/* Weights file (magic, layer table, fallback threshold, FP32 weights).
+ * save returns 0 / -1; load returns NULL on any mismatch. */
+int    slb_warm_save(const slb_warmnet *net, const char *path);
+slb_warmnet *slb_warm_load(const char *path);
+/* Add the lattice's current state (a converged solve) as a training sample
+ * held in device memory; returns the sample count.  sample_stats gives the
+ * mean square of the scaled velocity, density and stress targets. */
+int    slb_warm_add_sample(slb_warmnet *net, LBSys lb);
+int    slb_warm_nsamples(const slb_warmnet *net);
+void   slb_warm_sample_stats(const slb_warmnet *net, int i, double ms[3]);
+/* The network's inputs and targets for a lattice, written to a device BF16
+ * tensor [n][channels] (zeros beyond the used channels): features channel 0
+ * solid flag, 1 signed distance to the interface / 8 (periodic jump
+ * flooding: + on fluid, - on solid); targets the 10 moments above scaled
+ * by eta/(4F) (velocity), 1/(32F) (density), 1/F (stress). */
+void   slb_warm_features(LBSys lb, void *x_dev, int channels);
+void   slb_warm_targets(LBSys lb, void *t_dev, int channels);
+/* The same target moments in FP32 (device [n][channels]), and the inverse:
+ * populations from scaled moments m (device FP32 [n][channels >= 10]) in the
+ * given mode. */
+void   slb_warm_moments(LBSys lb, float *m_dev, int channels);
+void   slb_warm_inject(LBSys lb, const float *m_dev, int channels, int mode);
+/* iters Adam steps (cosine learning-rate decay from lr to lr/20) over
+ * seeded epochs of the samples; loss = mean over the three target groups of
+ * the relative mean-square error.  history (iters doubles) may be NULL.
+ * Returns the mean loss over the last min(iters, samples) steps. */
+double slb_warm_train(slb_warmnet *net, int iters, double lr, unsigned long long seed,
+                      double *history);
+/* Relative mean-square error of the prediction against the lattice's
+ * current state (group errors to grp[3] when non-NULL). */
+double slb_warm_eval(slb_warmnet *net, LBSys lb, double *grp);
+/* Predicted moments m[10n + c] (lattice units) of the lattice's solid map
+ * and body force. */
+void   slb_warm_predict(slb_warmnet *net, LBSys lb, float *m);
+void   slb_warm_set_mode(slb_warmnet *net, int mode);
+int    slb_warm_get_mode(const slb_warmnet *net);
+/* Initialise the populations from the prediction (the mode above). */
+void   slb_warm_start(slb_warmnet *net, LBSys lb);
+/* Cold-start fallback: after `probe` steps the warm residual must be below
+ * probe_res, or the populations are reset to rest and the solve restarts.
+ * note_cold records a cold solve's residual at the probe step and sets the
+ * threshold to the median of the recorded values. */
+void   slb_warm_set_fallback(slb_warmnet *net, int probe, double probe_res);
+void   slb_warm_get_fallback(const slb_warmnet *net, int *probe, double *probe_res);
+void   slb_warm_note_cold(slb_warmnet *net, double res_at_probe);
+/* Warm start, then run to slb_residual() < tol as slb_run_to_residual does,
+ * with the fallback.  Returns the total steps; *used_warm is 0 when the
+ * solve fell back to a cold start. */
+int    slb_warm_solve(slb_warmnet *net, LBSys lb, double tol, int interval, int maxsteps,
+                      double *res, int *used_warm);
+/* Parameter access for tests: the layer table, the parameter arena
+ * (W[co][Kp] then 64 bias slots per layer), and one forward+backward on
+ * device tensors x (dims d, 16 channels) and t (10 channels) with host loss
+ * weights wc[10], returning the weighted squared error, the FP32 output
+ * (host, [n][16]) and the gradient (host, nparams floats). */
+long   slb_warm_nparams(const slb_warmnet *net);
+void   slb_warm_layout(int *nlayer, int *cin, int *cout, long *wofs, long *bofs);
+void   slb_warm_get_params(const slb_warmnet *net, float *w);
+void   slb_warm_set_params(slb_warmnet *net, const float *w);
+double slb_warm_fwd_bwd(slb_warmnet *net, const int *d, const uint16_t *x, const uint16_t *t,
+                        const float *wc, float *outf, float *grad);
 
 #ifdef __cplusplus
 }

@@ -1,11 +1,20 @@
 /*
- * slb_nn.h -- internal C ABI of the warm-start network kernels (slb_nn.hip).
- * Tensors are device pointers, channels-last BF16 (uint16_t) with channel
- * counts that are multiples of 8; weights W[co][Kp], Kp = slbn_kpad(cin),
- * rows padded with zeros to a multiple of 16.
+ * sunlightlb_nn.h -- C ABI of the warm-start network kernels (gfx950 HIP,
+ * sunlightlb_amd/csrc/slb_nn.hip).  No counterpart in the reference: the
+ * reference has no learned component.  The host network (slb_warm.c) and
+ * the kernel tests call these.
+ *
+ * Tensors are device pointers, channels-last BF16 (uint16_t), X[n][c] with
+ * n = esk*(esj*i+j)+k, channel counts multiples of 8.  3x3x3 convolutions
+ * with periodic padding; tap t = 9*(di+1) + 3*(dj+1) + (dk+1) reads node
+ * (i+di, j+dj, k+dk).  Weights W[co][Kp] with k = t*cin + ci, Kp =
+ * slbn_kpad(cin) = 27*cin rounded up to 32, zero-filled (cout rows).  Every
+ * call is asynchronous on
+ * the stream set by slbn_set_stream (default: the null stream).
  */
-#ifndef SLB_NN_H
-#define SLB_NN_H
+#ifndef SUNLIGHTLB_NN_H_INC
+#define SUNLIGHTLB_NN_H_INC
+#include <stddef.h>
 #include <stdint.h>
 #ifdef __cplusplus
 extern "C" {
@@ -31,7 +40,9 @@ void slbn_pool_bwd(const uint16_t *dY, uint16_t *dX, int C, int D, int H, int Wd
 void slbn_up_add(const uint16_t *Xc, const uint16_t *S, uint16_t *Y, int C, int D, int H, int Wd);
 void slbn_up_bwd(const uint16_t *dY, uint16_t *dXc, int C, int D, int H, int Wd);
 void slbn_add_into(uint16_t *A, const uint16_t *B, long n);
-void slbn_mse(const float *Yf, int C, const uint16_t *T, int Ct, int Cv, int N, float scale,
+/* loss[0] += sum_n sum_{c<Cv} wc[c] (Yf[n][c] - T[n][c])^2 and dY[n][c] =
+ * 2 wc[c] (Yf - T) (0 for c >= Cv); wc is a device array of Cv floats */
+void slbn_mse(const float *Yf, int C, const uint16_t *T, int Ct, int Cv, int N, const float *wc,
               uint16_t *dY, float *loss);
 void slbn_adam(float *w, uint16_t *wb, const float *g, float *m, float *v, long n, float lr,
                float b1, float b2, float eps, int step);
@@ -39,6 +50,13 @@ void slbn_f2bf(const float *a, uint16_t *b, long n);
 void slbn_bf2f(const uint16_t *a, float *b, long n);
 /* expand a packed sample (Cs channels) into a C-channel tensor, zeros beyond Cs */
 void slbn_expand(const uint16_t *S, int Cs, uint16_t *X, int C, long N);
+/* device memory (exits with a message on failure) and stream sync */
+void *slbn_malloc(size_t bytes);
+void slbn_free(void *p);
+void slbn_zero(void *p, size_t bytes);
+void slbn_h2d(void *dst, const void *src, size_t bytes);
+void slbn_d2h(void *dst, const void *src, size_t bytes);
+void slbn_sync(void);
 #ifdef __cplusplus
 }
 #endif

@@ -76,6 +76,8 @@ MPSys = C.POINTER(MPSys_type)
 S = C.c_char_p
 D = C.c_double
 I = C.c_int
+VP = C.c_void_p
+LongP = C.POINTER(C.c_long)
 
 # name: (restype, argtypes) for every entry point of include/sunlightlb.h and
 # include/sunlightlb_gpu.h
@@ -172,6 +174,49 @@ SIGNATURES = {
     "slb_batch_get_pops": (I, [C.c_void_p, I, I, RealP]),
     "slb_batch_nodes": (C.c_long, [C.c_void_p]),
     "slb_batch_free": (None, [C.c_void_p]),
+    # learned warm start
+    "slb_warm_new": (C.c_void_p, [C.c_ulonglong]),
+    "slb_warm_free": (None, [C.c_void_p]),
+    "slb_warm_save": (I, [C.c_void_p, S]), "slb_warm_load": (C.c_void_p, [S]),
+    "slb_warm_add_sample": (I, [C.c_void_p, LBSys]),
+    "slb_warm_nsamples": (I, [C.c_void_p]),
+    "slb_warm_sample_stats": (None, [C.c_void_p, I, RealP]),
+    "slb_warm_set_mode": (None, [C.c_void_p, I]), "slb_warm_get_mode": (I, [C.c_void_p]),
+    "slb_warm_features": (None, [LBSys, VP, I]), "slb_warm_targets": (None, [LBSys, VP, I]),
+    "slb_warm_moments": (None, [LBSys, VP, I]), "slb_warm_inject": (None, [LBSys, VP, I, I]),
+    "slb_warm_train": (D, [C.c_void_p, I, D, C.c_ulonglong, RealP]),
+    "slb_warm_eval": (D, [C.c_void_p, LBSys, RealP]),
+    "slb_warm_predict": (None, [C.c_void_p, LBSys, VP]),
+    "slb_warm_start": (None, [C.c_void_p, LBSys]),
+    "slb_warm_set_fallback": (None, [C.c_void_p, I, D]),
+    "slb_warm_get_fallback": (None, [C.c_void_p, IntP, RealP]),
+    "slb_warm_note_cold": (None, [C.c_void_p, D]),
+    "slb_warm_solve": (I, [C.c_void_p, LBSys, D, I, I, RealP, IntP]),
+    "slb_warm_nparams": (C.c_long, [C.c_void_p]),
+    "slb_warm_layout": (None, [IntP, IntP, IntP, LongP, LongP]),
+    "slb_warm_get_params": (None, [C.c_void_p, VP]),
+    "slb_warm_set_params": (None, [C.c_void_p, VP]),
+    "slb_warm_fwd_bwd": (D, [C.c_void_p, IntP, VP, VP, VP, VP, VP]),
+    # network kernels (device pointers)
+    "slbn_kpad": (I, [I]), "slbn_set_stream": (None, [VP]),
+    "slbn_conv": (None, [VP, I, VP, VP, VP, VP, I, I, I, I, I, I]),
+    "slbn_conv_wgrad": (None, [VP, I, VP, I, VP, I, I, I]),
+    "slbn_conv_wgrad_small": (None, [VP, I, VP, I, VP, I, I, I]),
+    "slbn_wtrans": (None, [VP, I, I, VP]),
+    "slbn_relu_bwd": (None, [VP, VP, VP, C.c_long]),
+    "slbn_bias_grad": (None, [VP, I, I, VP]),
+    "slbn_pool": (None, [VP, VP, I, I, I, I]), "slbn_pool_bwd": (None, [VP, VP, I, I, I, I]),
+    "slbn_up_add": (None, [VP, VP, VP, I, I, I, I]), "slbn_up_bwd": (None, [VP, VP, I, I, I, I]),
+    "slbn_add_into": (None, [VP, VP, C.c_long]),
+    "slbn_mse": (None, [VP, I, VP, I, I, I, VP, VP, VP]),
+    "slbn_adam": (None, [VP, VP, VP, VP, VP, C.c_long, C.c_float, C.c_float, C.c_float,
+                         C.c_float, I]),
+    "slbn_f2bf": (None, [VP, VP, C.c_long]), "slbn_bf2f": (None, [VP, VP, C.c_long]),
+    "slbn_expand": (None, [VP, I, VP, I, C.c_long]),
+    "slbn_malloc": (VP, [C.c_size_t]), "slbn_free": (None, [VP]),
+    "slbn_zero": (None, [VP, C.c_size_t]),
+    "slbn_h2d": (None, [VP, VP, C.c_size_t]), "slbn_d2h": (None, [VP, VP, C.c_size_t]),
+    "slbn_sync": (None, []),
 }
 
 GLOBALS = ("slb_verbosity", "slb_signal_flag")

@@ -129,10 +129,17 @@ void slbd_dims(const slbd_lattice *L, int *d);
 /* channel 0 solid flag, channel 1 signed distance * scale (periodic jump
  * flooding); C-channel BF16 device tensor */
 void slbd_features(slbd_lattice *L, uint16_t *X, int C, float scale);
-/* channels 0..2: (j - b/2)/rho * scale on fluid nodes, 0 on solid */
-void slbd_targets(slbd_lattice *L, const slbd_params *P, float scale, uint16_t *Y, int C);
-/* equilibrium populations at rho0 and velocity U[n][0..2]*scale (fluid) */
-void slbd_inject(slbd_lattice *L, const float *U, int CU, double scale, double rho0, int stokes);
+/* the 10 moments of the live state on fluid nodes (0 on solid), scaled per
+ * group: 0..2 (j - b/2)/rho * sc[0], 3 drho * sc[1], 4..9 stress
+ * (xx,yy,zz,xy,xz,yz) * sc[2]; BF16 [n][C] */
+void slbd_targets(slbd_lattice *L, const slbd_params *P, const float *sc, uint16_t *Y, int C);
+/* the same in FP32 */
+void slbd_moments(slbd_lattice *L, const slbd_params *P, const float *sc, float *Y, int C);
+/* populations from predicted moments (the layout above) times inv_sc:
+ * mode 0 equilibrium at rho0 and u before a collide (lbloadvel), mode 1 the
+ * post-collide state with the predicted drho, u and stress */
+void slbd_inject(slbd_lattice *L, const slbd_params *P, const float *U, int CU,
+                 const double *inv_sc, int mode);
 
 /* ---- batched independent lattices (slb_batch.hip) ---- */
 typedef struct slbd_batch slbd_batch;

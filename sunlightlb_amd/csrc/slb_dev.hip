@@ -1514,61 +1514,103 @@ __global__ void k_features(const uint16_t *__restrict__ flags, const int *__rest
   }
 }
 
-/* target: corrected post-collide velocity (j - b/2)/rho * scale on fluid
- * nodes, 0 on solid nodes, channels 0..2 of a C-channel BF16 tensor */
-template <typename T>
+/* targets: the 10 hydrodynamic moments of the live (post-collide) state,
+ * scaled per group -- channels 0..2 corrected velocity (j - b/2)/rho *
+ * sc[0], channel 3 drho * sc[1], channels 4..9 the stress s = Pi - drho/3 I
+ * (xx, yy, zz, xy, xz, yz) * sc[2] -- on fluid nodes, 0 on solid nodes;
+ * channels from 10 to C-1 are zero (C < 10 keeps the first C). */
+template <typename T, typename O>
 __global__ void k_targets(const T *__restrict__ a, const uint16_t *__restrict__ flags, double rho0,
-                          double bx, double by, double bz, float scale, uint16_t *__restrict__ Y,
-                          int C, Geom G) {
+                          double bx, double by, double bz, float su, float sr, float ss,
+                          O *__restrict__ Y, int C, Geom G) {
   const long n = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= G.N) return;
-  float u[3] = {0.f, 0.f, 0.f};
+  float m[10];
+#pragma unroll
+  for (int q = 0; q < 10; q++) m[q] = 0.f;
   if (!(flags[n] & FLAG_SOLID)) {
-    double d, jx, jy, jz;
-    node_rho_j<T>(a, n, G.N, d, jx, jy, jz);
+    double d = 0, j[3] = {0, 0, 0}, P[6] = {0, 0, 0, 0, 0, 0};
+#pragma unroll
+    for (int l = 0; l < 15; l++) {
+      const double v = (double)a[(long)l * G.N + n];
+      d += v;
+      j[0] += CX(l) * v; j[1] += CY(l) * v; j[2] += CZ(l) * v;
+      P[0] += CX(l) * CX(l) * v; P[1] += CY(l) * CY(l) * v; P[2] += CZ(l) * CZ(l) * v;
+      P[3] += CX(l) * CY(l) * v; P[4] += CX(l) * CZ(l) * v; P[5] += CY(l) * CZ(l) * v;
+    }
     const double rho = rho0 + d;
-    u[0] = (float)((jx - 0.5 * bx) / rho * scale);
-    u[1] = (float)((jy - 0.5 * by) / rho * scale);
-    u[2] = (float)((jz - 0.5 * bz) / rho * scale);
+    m[0] = (float)((j[0] - 0.5 * bx) / rho * su);
+    m[1] = (float)((j[1] - 0.5 * by) / rho * su);
+    m[2] = (float)((j[2] - 0.5 * bz) / rho * su);
+    m[3] = (float)(d * sr);
+    m[4] = (float)((P[0] - d / 3.0) * ss);
+    m[5] = (float)((P[1] - d / 3.0) * ss);
+    m[6] = (float)((P[2] - d / 3.0) * ss);
+    m[7] = (float)(P[3] * ss);
+    m[8] = (float)(P[4] * ss);
+    m[9] = (float)(P[5] * ss);
   }
   for (int c = 0; c < C; c++) {
-    const float v = c < 3 ? u[c] : 0.f;
-    const uint32_t w = __float_as_uint(v) + 0x7fffu + ((__float_as_uint(v) >> 16) & 1u);
-    Y[n * C + c] = (uint16_t)(w >> 16);
+    const float v = c < 10 ? m[c] : 0.f;
+    if constexpr (sizeof(O) == 4) {
+      Y[n * C + c] = v;
+    } else {
+      const uint32_t w = __float_as_uint(v) + 0x7fffu + ((__float_as_uint(v) >> 16) & 1u);
+      Y[n * C + c] = (uint16_t)(w >> 16);
+    }
   }
 }
 
-/* equilibrium populations at density rho0 and velocity u = U[n][0..2]*scale
- * (fluid nodes; u = 0 on solid nodes): drho = 0, j = rho0 u, stress 0
- * (Stokes) or u j (the rebuild of lbloadvel, src/lbstore.c:346-399) */
+/* Populations from predicted moments U[n][0..9] (the layout of k_targets)
+ * divided by the group scales (iu, ir, is = 1/sc):
+ *   mode 0 -- the lbloadvel rebuild (src/lbstore.c:346-399): drho = 0,
+ *             j = rho0 u - b/2 on fluid nodes (so that the collide that
+ *             follows, adding b, leaves the corrected velocity u), stress 0
+ *             (Stokes) or u j; rest on solid nodes;
+ *   mode 1 -- the post-collide state itself: drho, j = (rho0 + drho) u + b/2
+ *             and the predicted stress on fluid nodes, rest on solid nodes.
+ * Both use the node algebra of collide: a_l = (w_l/72)(drho + 3 c.j +
+ * 9/2 (cc:s - tr s / 3)). */
 template <typename T>
 __global__ void k_inject(T *__restrict__ a, const uint16_t *__restrict__ flags,
-                         const float *__restrict__ U, int CU, double scale, double rho0, int stokes,
+                         const float *__restrict__ U, int CU, double iu, double ir, double is,
+                         double rho0, double bx, double by, double bz, int stokes, int mode,
                          Geom G) {
   const long n = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= G.N) return;
-  double u[3] = {0.0, 0.0, 0.0};
-  if (!(flags[n] & FLAG_SOLID))
-    for (int c = 0; c < 3; c++) u[c] = (double)U[n * CU + c] * scale;
-  const double jx = rho0 * u[0], jy = rho0 * u[1], jz = rho0 * u[2];
-  double sxx = 0, syy = 0, szz = 0, sxy = 0, sxz = 0, syz = 0;
-  if (!stokes) {
-    sxx = u[0] * jx; syy = u[1] * jy; szz = u[2] * jz;
-    sxy = u[0] * jy; sxz = u[0] * jz; syz = u[1] * jz;
+  double d = 0.0, j[3] = {0.0, 0.0, 0.0};
+  double s[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+  if (!(flags[n] & FLAG_SOLID)) {
+    const float *u = U + n * CU;
+    if (mode == 0) {
+      j[0] = rho0 * (u[0] * iu) - 0.5 * bx;
+      j[1] = rho0 * (u[1] * iu) - 0.5 * by;
+      j[2] = rho0 * (u[2] * iu) - 0.5 * bz;
+      if (!stokes)
+        for (int p = 0; p < 3; p++)
+          for (int q = 0; q < 3; q++) s[p][q] = j[p] * j[q] / rho0;
+    } else {
+      d = u[3] * ir;
+      const double rho = rho0 + d;
+      j[0] = rho * (u[0] * iu) + 0.5 * bx;
+      j[1] = rho * (u[1] * iu) + 0.5 * by;
+      j[2] = rho * (u[2] * iu) + 0.5 * bz;
+      s[0][0] = u[4] * is; s[1][1] = u[5] * is; s[2][2] = u[6] * is;
+      s[0][1] = s[1][0] = u[7] * is;
+      s[0][2] = s[2][0] = u[8] * is;
+      s[1][2] = s[2][1] = u[9] * is;
+    }
   }
-  const double t3 = (sxx + syy + szz) / 3.0;
-  double v[15];
-  v[0] = -t3;
-  double q, g;
-  q = 0.5 * (sxx - t3); g = jx / 3.0; v[1] = q + g; v[2] = q - g;
-  q = 0.5 * (syy - t3); g = jy / 3.0; v[3] = q + g; v[4] = q - g;
-  q = 0.5 * (szz - t3); g = jz / 3.0; v[5] = q + g; v[6] = q - g;
-  q = 0.125 * (t3 + sxy + sxz + syz); g = (jx + jy + jz) / 24.0; v[7] = q + g; v[14] = q - g;
-  q = 0.125 * (t3 - sxy - sxz + syz); g = (-jx + jy + jz) / 24.0; v[8] = q + g; v[13] = q - g;
-  q = 0.125 * (t3 - sxy + sxz - syz); g = (jx - jy + jz) / 24.0; v[9] = q + g; v[12] = q - g;
-  q = 0.125 * (t3 + sxy - sxz - syz); g = (-jx - jy + jz) / 24.0; v[10] = q + g; v[11] = q - g;
+  const double tr = s[0][0] + s[1][1] + s[2][2];
 #pragma unroll
-  for (int l = 0; l < 15; l++) a[(long)l * G.N + n] = (T)v[l];
+  for (int l = 0; l < 15; l++) {
+    const int c[3] = {CX(l), CY(l), CZ(l)};
+    double cs = 0.0;
+    for (int p = 0; p < 3; p++)
+      for (int q = 0; q < 3; q++) cs += c[p] * c[q] * s[p][q];
+    const double cj = c[0] * j[0] + c[1] * j[1] + c[2] * j[2];
+    a[(long)l * G.N + n] = (T)((WL(l) / 72.0) * (d + 3.0 * cj + 4.5 * (cs - tr / 3.0)));
+  }
 }
 
 extern "C" long slbd_nodes(const slbd_lattice *L) { return L->N; }
@@ -1606,26 +1648,43 @@ extern "C" void slbd_features(slbd_lattice *L, uint16_t *X, int C, float scale) 
   for (int b = 0; b < 3; b++) HIPCHECK(hipFree(buf[b]));
 }
 
-extern "C" void slbd_targets(slbd_lattice *L, const slbd_params *P, float scale, uint16_t *Y, int C) {
+template <typename O>
+static void launch_targets(slbd_lattice *L, const slbd_params *P, const float *sc, O *Y, int C) {
   const double bx = P->bf[0], by = P->bf[1], bz = P->bf[2];
   if (L->prec == 32)
-    hipLaunchKernelGGL(k_targets<float>, lin_grid(L->N), dim3(256), 0, L->stream,
-                       (const float *)L->pops[L->cur], L->flags, P->rho0, bx, by, bz, scale, Y, C, L->G);
+    hipLaunchKernelGGL((k_targets<float, O>), lin_grid(L->N), dim3(256), 0, L->stream,
+                       (const float *)L->pops[L->cur], L->flags, P->rho0, bx, by, bz, sc[0], sc[1],
+                       sc[2], Y, C, L->G);
   else
-    hipLaunchKernelGGL(k_targets<double>, lin_grid(L->N), dim3(256), 0, L->stream,
-                       (const double *)L->pops[L->cur], L->flags, P->rho0, bx, by, bz, scale, Y, C, L->G);
+    hipLaunchKernelGGL((k_targets<double, O>), lin_grid(L->N), dim3(256), 0, L->stream,
+                       (const double *)L->pops[L->cur], L->flags, P->rho0, bx, by, bz, sc[0], sc[1],
+                       sc[2], Y, C, L->G);
   LAUNCHCHECK();
   HIPCHECK(hipStreamSynchronize(L->stream));
 }
 
-extern "C" void slbd_inject(slbd_lattice *L, const float *U, int CU, double scale, double rho0,
-                            int stokes) {
+extern "C" void slbd_targets(slbd_lattice *L, const slbd_params *P, const float *sc, uint16_t *Y,
+                             int C) {
+  launch_targets<uint16_t>(L, P, sc, Y, C);
+}
+
+extern "C" void slbd_moments(slbd_lattice *L, const slbd_params *P, const float *sc, float *Y,
+                             int C) {
+  launch_targets<float>(L, P, sc, Y, C);
+}
+
+extern "C" void slbd_inject(slbd_lattice *L, const slbd_params *P, const float *U, int CU,
+                            const double *inv_sc, int mode) {
+  const double rho0 = P->rho0, bx = P->bf[0], by = P->bf[1], bz = P->bf[2];
+  const int st = P->stokes_flow;
   if (L->prec == 32)
     hipLaunchKernelGGL(k_inject<float>, lin_grid(L->N), dim3(256), 0, L->stream,
-                       (float *)L->pops[L->cur], L->flags, U, CU, scale, rho0, stokes, L->G);
+                       (float *)L->pops[L->cur], L->flags, U, CU, inv_sc[0], inv_sc[1], inv_sc[2],
+                       rho0, bx, by, bz, st, mode, L->G);
   else
     hipLaunchKernelGGL(k_inject<double>, lin_grid(L->N), dim3(256), 0, L->stream,
-                       (double *)L->pops[L->cur], L->flags, U, CU, scale, rho0, stokes, L->G);
+                       (double *)L->pops[L->cur], L->flags, U, CU, inv_sc[0], inv_sc[1], inv_sc[2],
+                       rho0, bx, by, bz, st, mode, L->G);
   LAUNCHCHECK();
   HIPCHECK(hipStreamSynchronize(L->stream));
 }

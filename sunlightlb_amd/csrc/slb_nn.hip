@@ -23,7 +23,7 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
-#include "slb_nn.h"
+#include "sunlightlb_nn.h"
 
 #define HIPCHECK(call)                                                        \
   do {                                                                        \
@@ -104,7 +104,8 @@ k_conv(const uint16_t *__restrict__ X, int Cin, const uint16_t *__restrict__ W, 
 #pragma unroll
     for (int m = 0; m < MT; m++) {
       const int co = 16 * m + (lane & 15);
-      const bf16x8 a = *(const bf16x8 *)(W + (long)co * Kp + kk);
+      bf16x8 a = (bf16x8){0, 0, 0, 0, 0, 0, 0, 0};
+      if (co < Cout) a = *(const bf16x8 *)(W + (long)co * Kp + kk);
       acc[m] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[m], 0, 0, 0);
     }
   }
@@ -294,12 +295,13 @@ __global__ void k_add_into(uint16_t *__restrict__ A, const uint16_t *__restrict_
   A[e] = f2bf(bf2f(A[e]) + bf2f(B[e]));
 }
 
-/* Weighted mean-square error of an FP32 prediction against a BF16 target
- * over the first Cv channels, gradient into dY (BF16, C channels): the sum
- * goes to loss[0] via atomics. */
+/* Weighted squared error of an FP32 prediction against a BF16 target over
+ * the first Cv channels, sum_n sum_c wc[c] (Yf - T)^2 into loss[0] (atomics),
+ * and its gradient 2 wc[c] (Yf - T) into dY (BF16, C channels, zero beyond
+ * Cv).  wc: Cv FP32 weights in device memory. */
 __global__ void __launch_bounds__(256)
 k_mse(const float *__restrict__ Yf, int C, const uint16_t *__restrict__ T, int Ct, int Cv,
-      int N, float scale, uint16_t *__restrict__ dY, float *__restrict__ loss) {
+      int N, const float *__restrict__ wc, uint16_t *__restrict__ dY, float *__restrict__ loss) {
   const int n = blockIdx.x * blockDim.x + threadIdx.x;
   float s = 0.f;
   if (n < N) {
@@ -307,8 +309,8 @@ k_mse(const float *__restrict__ Yf, int C, const uint16_t *__restrict__ T, int C
       float g = 0.f;
       if (c < Cv) {
         const float d = Yf[(long)n * C + c] - bf2f(T[(long)n * Ct + c]);
-        s += d * d;
-        g = 2.f * d * scale;
+        s += wc[c] * d * d;
+        g = 2.f * d * wc[c];
       }
       dY[(long)n * C + c] = f2bf(g);
     }
@@ -366,6 +368,10 @@ extern "C" void slbn_conv(const uint16_t *X, int cin, const uint16_t *W, const f
   const int Kp = slbn_kpad(cin);
   const unsigned g = blocks_for(G.N, 64);
   const int sgn = transposed ? -1 : 1;
+  if (cin < 8 || cin % 8 || cout < 1 || cout % 4 || (long)G.N * (cin > cout ? cin : cout) >= (1L << 31)) {
+    fprintf(stderr, "slbn_conv: cin must be a multiple of 8, cout of 4, tensors < 2^31 elements\n");
+    exit(EXIT_FAILURE);
+  }
 #define SLBN_CONV(MT)                                                                      \
   hipLaunchKernelGGL(k_conv<MT>, dim3(g), dim3(256), 0, nn_stream, X, cin, W, Kp, bias, Y, Yf, \
                      cout, relu, sgn, G)
@@ -418,6 +424,10 @@ extern "C" void slbn_relu_bwd(const uint16_t *Y, const uint16_t *dY, uint16_t *d
 }
 
 extern "C" void slbn_bias_grad(const uint16_t *dP, int C, int N, float *db) {
+  if (C < 1 || C > 64) {
+    fprintf(stderr, "slbn_bias_grad: 1..64 channels\n");
+    exit(EXIT_FAILURE);
+  }
   hipLaunchKernelGGL(k_bias_grad, dim3(256), dim3(256), 0, nn_stream, dP, C, N, db);
   LAUNCHCHECK();
 }
@@ -452,9 +462,9 @@ extern "C" void slbn_add_into(uint16_t *A, const uint16_t *B, long n) {
   LAUNCHCHECK();
 }
 extern "C" void slbn_mse(const float *Yf, int C, const uint16_t *T, int Ct, int Cv, int N,
-                         float scale, uint16_t *dY, float *loss) {
+                         const float *wc, uint16_t *dY, float *loss) {
   hipLaunchKernelGGL(k_mse, dim3(blocks_for(N, 256)), dim3(256), 0, nn_stream, Yf, C, T, Ct, Cv, N,
-                     scale, dY, loss);
+                     wc, dY, loss);
   LAUNCHCHECK();
 }
 extern "C" void slbn_adam(float *w, uint16_t *wb, const float *g, float *m, float *v, long n,
@@ -516,3 +526,28 @@ extern "C" void slbn_expand(const uint16_t *S, int Cs, uint16_t *X, int C, long 
   hipLaunchKernelGGL(k_expand, dim3(blocks_for(N * C, 256)), dim3(256), 0, nn_stream, S, Cs, X, C, N);
   LAUNCHCHECK();
 }
+
+/* ---------------------------------------------------------------------- */
+/* Device memory helpers for the host-side network code (plain C).          */
+
+extern "C" void *slbn_malloc(size_t bytes) {
+  void *p = nullptr;
+  if (bytes == 0) bytes = 256;
+  HIPCHECK(hipMalloc(&p, bytes));
+  return p;
+}
+extern "C" void slbn_free(void *p) {
+  if (p) HIPCHECK(hipFree(p));
+}
+extern "C" void slbn_zero(void *p, size_t bytes) {
+  HIPCHECK(hipMemsetAsync(p, 0, bytes, nn_stream));
+}
+extern "C" void slbn_h2d(void *dst, const void *src, size_t bytes) {
+  HIPCHECK(hipStreamSynchronize(nn_stream));
+  HIPCHECK(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+}
+extern "C" void slbn_d2h(void *dst, const void *src, size_t bytes) {
+  HIPCHECK(hipStreamSynchronize(nn_stream));
+  HIPCHECK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+}
+extern "C" void slbn_sync(void) { HIPCHECK(hipStreamSynchronize(nn_stream)); }

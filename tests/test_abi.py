@@ -37,7 +37,7 @@ def exported_symbols(path):
 def test_library_exports_every_declared_symbol():
     assert os.path.exists(LIB), "build the library first (__graft_entry__.build())"
     syms = exported_symbols(LIB)
-    for h in ("sunlightlb.h", "sunlightlb_gpu.h"):
+    for h in ("sunlightlb.h", "sunlightlb_gpu.h", "sunlightlb_nn.h"):
         decl = declared_functions(os.path.join(INC, h))
         assert len(decl) > 20 or h != "sunlightlb.h"
         missing = [d for d in decl if d not in syms]
@@ -50,6 +50,7 @@ def test_python_binding_declares_every_symbol():
     import sunlightlb_amd
     decl = set(declared_functions(os.path.join(INC, "sunlightlb.h")))
     decl |= set(declared_functions(os.path.join(INC, "sunlightlb_gpu.h")))
+    decl |= set(declared_functions(os.path.join(INC, "sunlightlb_nn.h")))
     assert decl <= set(sunlightlb_amd.SIGNATURES), decl - set(sunlightlb_amd.SIGNATURES)
     L = sunlightlb_amd.lib()  # loading needs no GPU
     assert L.myscmp(b"null", b"nullfile") == 1 and L.myscmp(b"abc", b"abd") == 0
