@@ -313,6 +313,214 @@ def batch_workload(L, args, u0=1e-3, tau=1.0):
     return res
 
 
+def warm_params(rng, count):
+    """Seeded pack parameters: radius U[4, 10], lattice solid fraction
+    U[0.2, 0.45]; random sequential addition up to 0.34, overlapping spheres
+    (Boolean model) above, where sequential addition jams."""
+    out = []
+    for _ in range(count):
+        R = float(rng.uniform(4.0, 10.0))
+        phi = float(rng.uniform(0.2, 0.45))
+        out.append((R, phi, int(phi > 0.34), int(rng.integers(1, 2 ** 62))))
+    return out
+
+
+def warm_maps(L, n, params, workers):
+    """Solid maps of the packs, generated on host threads (slb_pack_map is
+    thread-safe and ctypes releases the GIL), yielded in order."""
+    import numpy as np
+    from concurrent.futures import ThreadPoolExecutor
+
+    def one(p):
+        R, phi, ov, seed = p
+        m = np.empty(n ** 3, np.uint8)
+        po = C.c_double(0.0)
+        L.slb_pack_map(n, n, n, R, phi, seed, ov, 50_000_000, m.ctypes.data, C.byref(po))
+        return m, po.value
+
+    with ThreadPoolExecutor(max_workers=workers) as ex:
+        yield from ex.map(one, params)
+
+
+def warm_lattice(L, n, solid, force, precision=32):
+    h = L.newlb(n, n, n, b"null")
+    st = h.contents
+    st.stokes_flow = 1
+    st.tau = 0.75
+    L.slb_insert_map(h, solid.ctypes.data)
+    L.zerobf(h)
+    L.addbf(h, force, 0.0, 0.0)
+    L.syncobjs(h)
+    L.slb_set_precision(h, precision)
+    return h
+
+
+def solve_loop(L, h, tol, interval, maxsteps):
+    """Run to slb_residual() < tol in chunks of `interval` (as
+    slb_run_to_residual); returns steps, the residual history and the final
+    residual."""
+    L.slb_residual_reset(h)
+    L.slb_residual(h)
+    steps, hist, r = 0, [], -1.0
+    while steps < maxsteps:
+        L.slb_run(h, interval)
+        steps += interval
+        r = L.slb_residual(h)
+        hist.append(r)
+        if 0.0 <= r < tol:
+            break
+    return steps, hist, r
+
+
+def warm_workload(L, args):
+    """Learned warm start: generate and solve training packs on the GPU,
+    train the network on them, then solve unseen packs cold and warm to the
+    same residual and compare steps, wall time and permeability."""
+    import numpy as np
+    rng = np.random.default_rng(args.seed + 5)
+    n = args.warm_size
+    F = args.force
+    eta = 72.0 * (0.75 - 0.5) / 3.0
+    workers = max(1, min(16, os.cpu_count() or 1))
+    net = L.slb_warm_new(args.seed)
+    res = {"train_packs": args.warm_train, "train_size": n, "workers": workers}
+    # ---- training data: cold solves of seeded packs ----
+    t0 = time.perf_counter()
+    cold_steps = []
+    for i, (solid, phi) in enumerate(warm_maps(L, n, warm_params(rng, args.warm_train), workers)):
+        h = warm_lattice(L, n, solid, F)
+        steps, hist, r = solve_loop(L, h, args.tol, args.interval, args.maxsteps)
+        L.slb_warm_note_cold(net, (C.c_double * len(hist))(*hist), len(hist), args.interval)
+        cold_steps.append(steps)
+        L.slb_warm_add_sample(net, h)
+        L.slb_destroy(h)
+        if (i + 1) % max(1, args.warm_train // 10) == 0:
+            log(f"warm: {i + 1}/{args.warm_train} training solves, {time.perf_counter() - t0:.0f} s "
+                f"(median {int(np.median(cold_steps))} steps)")
+    res["data_seconds"] = round(time.perf_counter() - t0, 1)
+    res["train_cold_steps_median"] = float(np.median(cold_steps))
+    # ---- training ----
+    hist = (C.c_double * args.warm_iters)()
+    t0 = time.perf_counter()
+    last = L.slb_warm_train(net, args.warm_iters, args.warm_lr, args.seed + 7, hist)
+    dt = time.perf_counter() - t0
+    hv = np.array(hist[:])
+    k = max(1, args.warm_iters // 20)
+    curve = [float(hv[i:i + k].mean()) for i in range(0, args.warm_iters, k)]
+    log(f"warm: trained {args.warm_iters} iterations in {dt:.1f} s ({dt / args.warm_iters * 1e3:.2f} "
+        f"ms/iter), loss {curve[0]:.3e} -> {last:.3e}")
+    pr, pres = C.c_int(0), C.c_double(0.0)
+    L.slb_warm_get_fallback(net, C.byref(pr), C.byref(pres))
+    res.update({"train_iters": args.warm_iters, "train_seconds": round(dt, 1),
+                "ms_per_iter": round(dt / args.warm_iters * 1e3, 3), "loss_curve": curve,
+                "final_loss": last, "fallback_probe_steps": pr.value,
+                "fallback_probe_residual": pres.value})
+    if args.warm_save:
+        L.slb_warm_save(net, args.warm_save.encode())
+    # ---- evaluation on unseen packs ----
+    sizes = [int(x) for x in args.warm_eval_sizes.split(",")]
+    rows = []
+    erng = np.random.default_rng(args.seed + 11)
+    for size in sizes:
+        params = warm_params(erng, args.warm_eval // len(sizes))
+        for (solid, phi), p in zip(warm_maps(L, size, params, workers), params):
+            row = {"n": size, "R": round(p[0], 3), "phi": round(phi, 4), "overlap": p[2]}
+            first = len([r for r in rows if r["n"] == size]) < 4
+            for mode in ("cold", "warm", "warm_trace", "equilibrium_trace"):
+                if mode == "equilibrium_trace" and not first:
+                    continue
+                h = warm_lattice(L, size, solid, F)
+                L.slb_residual(h)  # uploads the geometry before the clock starts
+                hist = None
+                t0 = time.perf_counter()
+                if mode == "cold":
+                    steps, hist, r = solve_loop(L, h, args.tol, args.interval, args.maxsteps)
+                    used = 0
+                elif mode == "warm":
+                    L.slb_warm_set_mode(net, 1)
+                    rr, uw = C.c_double(0.0), C.c_int(0)
+                    steps = L.slb_warm_solve(net, h, args.tol, args.interval, args.maxsteps,
+                                             C.byref(rr), C.byref(uw))
+                    r, used = rr.value, uw.value
+                else:
+                    # the warm start without the fallback, for its residual history
+                    L.slb_warm_set_mode(net, 1 if mode == "warm_trace" else 0)
+                    L.slb_warm_start(net, h)
+                    steps, hist, r = solve_loop(L, h, args.tol, args.interval, args.maxsteps)
+                    used = 1
+                L.measure(h)
+                dt = time.perf_counter() - t0
+                kperm = h.contents.vmeanx * eta / F
+                row[mode] = {"steps": steps, "seconds": round(dt, 4), "residual": r,
+                             "permeability": kperm, "used_warm": used}
+                if hist is not None:
+                    for tol in (1e-3, 1e-4, 1e-5, 1e-6):
+                        k = next((i for i, x in enumerate(hist) if 0.0 <= x < tol), None)
+                        row[mode][f"steps_to_{tol:.0e}"] = (None if k is None
+                                                            else (k + 1) * args.interval)
+                if mode == "cold":
+                    grp = (C.c_double * 3)()
+                    row["pred_rel_mse"] = L.slb_warm_eval(net, h, grp)
+                    row["pred_rel_mse_groups"] = list(grp)
+                L.slb_destroy(h)
+            for mode in ("warm", "warm_trace", "equilibrium_trace"):
+                if mode in row:
+                    row[mode]["k_rel_diff"] = (abs(row[mode]["permeability"] - row["cold"]["permeability"])
+                                               / abs(row["cold"]["permeability"]))
+            wt, ct = row["warm_trace"], row["cold"]
+            log(f"warm eval n={size} R={p[0]:.2f} phi={phi:.3f}: cold {ct['steps']} steps "
+                f"{ct['seconds']:.3f} s, warm {row['warm']['steps']} steps "
+                f"{row['warm']['seconds']:.3f} s (kept warm: {row['warm']['used_warm']}), "
+                f"to 1e-3/1e-4/1e-5 cold {ct['steps_to_1e-03']}/{ct['steps_to_1e-04']}/"
+                f"{ct['steps_to_1e-05']} warm {wt['steps_to_1e-03']}/{wt['steps_to_1e-04']}/"
+                f"{wt['steps_to_1e-05']}, pred err u/rho/s "
+                + "/".join(f"{x:.2e}" for x in row["pred_rel_mse_groups"]))
+            rows.append(row)
+            log(f"warm eval n={size} R={p[0]:.2f} phi={phi:.3f}: cold {row['cold']['steps']} steps "
+                f"{row['cold']['seconds']:.3f} s, warm {row['warm']['steps']} steps "
+                f"{row['warm']['seconds']:.3f} s (used {row['warm']['used_warm']}), "
+                f"dk/k {row['warm']['k_rel_diff']:.1e}, pred err u/rho/s "
+                + "/".join(f"{x:.2e}" for x in row["pred_rel_mse_groups"])
+                + (f", equilibrium {row['equilibrium']['steps']} steps" if "equilibrium" in row else ""))
+    L.slb_warm_free(net)
+    summ = {}
+    for size in sizes:
+        rs = [r for r in rows if r["n"] == size]
+        cs = np.array([r["cold"]["seconds"] for r in rs])
+        ws = np.array([r["warm"]["seconds"] for r in rs])
+        cst = np.array([r["cold"]["steps"] for r in rs])
+        wst = np.array([r["warm"]["steps"] for r in rs])
+        dk = np.array([r["warm"]["k_rel_diff"] for r in rs])
+        dkt = np.array([r["warm_trace"]["k_rel_diff"] for r in rs])
+        eq = [r for r in rs if "equilibrium_trace" in r]
+
+        def ratio(key, mode, sub=rs):
+            c = [r["cold"][key] for r in sub]
+            w = [r[mode][key] for r in sub]
+            if any(x is None for x in c + w):
+                return None
+            return float(sum(c) / sum(w))
+
+        summ[str(size)] = {
+            "packs": len(rs),
+            "cold_steps_mean": float(cst.mean()), "warm_steps_mean": float(wst.mean()),
+            "cold_seconds_total": float(cs.sum()), "warm_seconds_total": float(ws.sum()),
+            "wall_time_ratio_cold_over_warm": float(cs.sum() / ws.sum()),
+            "steps_ratio_cold_over_warm": float(cst.sum() / wst.sum()),
+            "fell_back_to_cold": int(sum(1 for r in rs if r["warm"]["used_warm"] == 0)),
+            "k_rel_diff_max": float(dk.max()), "k_rel_diff_median": float(np.median(dk)),
+            "no_fallback_k_rel_diff_max": float(dkt.max()),
+            "no_fallback_steps_ratio": {t: ratio(f"steps_to_{t}", "warm_trace")
+                                        for t in ("1e-03", "1e-04", "1e-05", "1e-06")},
+            "equilibrium_steps_ratio": {t: ratio(f"steps_to_{t}", "equilibrium_trace", eq)
+                                        for t in ("1e-03", "1e-04", "1e-05", "1e-06")} if eq else None,
+            "pred_rel_mse_mean": float(np.mean([r["pred_rel_mse"] for r in rs])),
+        }
+    res["eval"] = summ
+    res["eval_rows"] = rows
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -341,9 +549,25 @@ def main():
                          "right-hand sides in one launch per step)")
     ap.add_argument("--batch-boxes", type=int, default=512)
     ap.add_argument("--batch-steps", type=int, default=200)
+    ap.add_argument("--warm", action="store_true",
+                    help="learned warm start: training data, training, warm vs cold evaluation")
+    ap.add_argument("--warm-only", action="store_true",
+                    help="run only the warm-start workload (no step-throughput line)")
+    ap.add_argument("--warm-size", type=int, default=128)
+    ap.add_argument("--warm-train", type=int, default=2000)
+    ap.add_argument("--warm-iters", type=int, default=8000)
+    ap.add_argument("--warm-lr", type=float, default=1e-3)
+    ap.add_argument("--warm-eval", type=int, default=64)
+    ap.add_argument("--warm-eval-sizes", default="128,256")
+    ap.add_argument("--warm-save", default="")
     ap.add_argument("--copy-test", action="store_true",
                     help="only run the plain device copy kernel (counter calibration)")
     args = ap.parse_args()
+    # the one JSON line goes to the original stdout; everything the library
+    # prints (newlb banners, lbloadvel notes) goes to stderr
+    json_out = os.fdopen(os.dup(1), "w")
+    sys.stdout.flush()
+    os.dup2(2, 1)
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on exactly one GPU: the lattice-Boltzmann path here is "
@@ -360,6 +584,9 @@ def main():
     log(f"device copy: {copy_bytes >> 20} MiB in {copy_ms * 1e3:.1f} us, {copy_gbs:.0f} GB/s "
         f"(read + write)")
     if args.copy_test:
+        return
+    if args.warm_only:
+        print(json.dumps({"warm": warm_workload(L, args)}), file=json_out, flush=True)
         return
     n = args.size
     h, nsph, phi = build_pack(L, n, args.radius, args.phi, args.seed, args.force)
@@ -422,7 +649,9 @@ def main():
         out["mp"] = mp_workload(L, args)
     if args.batch:
         out["batch"] = batch_workload(L, args)
-    print(json.dumps(out), flush=True)
+    if args.warm:
+        out["warm"] = warm_workload(L, args)
+    print(json.dumps(out), file=json_out, flush=True)
 
 
 if __name__ == "__main__":

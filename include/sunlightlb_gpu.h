@@ -91,6 +91,18 @@ int slb_insert_spheres(LBSys lb, int n, const double *rxyz);
 int slb_rsa_pack(LBSys lb, double R, double phi, unsigned long long seed,
                  long maxtrials, double *centres, int maxcentres, double *phi_out);
 
+/* The solid map (solid[n] = 1, n = esk*(esj*i+j)+k) of a periodic sphere
+ * pack of radius R to lattice solid fraction phi, without a lattice:
+ * overlap = 0 is the random sequential addition of slb_rsa_pack (same
+ * seed, same spheres, same map as slb_rsa_pack followed by flattenobj);
+ * overlap = 1 places spheres at random without an overlap test (a Boolean
+ * model, for fractions RSA cannot reach).  Thread-safe.  Returns the spheres
+ * placed (-1 on allocation failure); *phi_out the achieved fraction. */
+int slb_pack_map(int esi, int esj, int esk, double R, double phi, unsigned long long seed,
+                 int overlap, long maxtrials, unsigned char *solid, double *phi_out);
+/* Insert the nodes with solid[n] != 0 as one object (NULL if none). */
+LBObj slb_insert_map(LBSys lb, const unsigned char *solid);
+
 /* ---- moment propagation extensions ---- */
 
 /* Partial adsorption: each boundary solid node without the 0x4000 bit keeps
@@ -196,22 +208,23 @@ int    slb_warm_save(const slb_warmnet *net, const char *path);
 slb_warmnet *slb_warm_load(const char *path);
 /* Add the lattice's current state (a converged solve) as a training sample
  * held in device memory; returns the sample count.  sample_stats gives the
- * mean square of the scaled velocity, density and stress targets. */
+ * mean square of the scaled velocity, density and stress targets and the
+ * sample's normalising length ell. */
 int    slb_warm_add_sample(slb_warmnet *net, LBSys lb);
 int    slb_warm_nsamples(const slb_warmnet *net);
-void   slb_warm_sample_stats(const slb_warmnet *net, int i, double ms[3]);
+void   slb_warm_sample_stats(const slb_warmnet *net, int i, double out[4]);
 /* The network's inputs and targets for a lattice, written to a device BF16
- * tensor [n][channels] (zeros beyond the used channels): features channel 0
- * solid flag, 1 signed distance to the interface / 8 (periodic jump
- * flooding: + on fluid, - on solid); targets the 10 moments above scaled
- * by eta/(4F) (velocity), 1/(32F) (density), 1/F (stress). */
-void   slb_warm_features(LBSys lb, void *x_dev, int channels);
-void   slb_warm_targets(LBSys lb, void *t_dev, int channels);
-/* The same target moments in FP32 (device [n][channels]), and the inverse:
- * populations from scaled moments m (device FP32 [n][channels >= 10]) in the
- * given mode. */
-void   slb_warm_moments(LBSys lb, float *m_dev, int channels);
-void   slb_warm_inject(LBSys lb, const float *m_dev, int channels, int mode);
+ * tensor [n][channels] (zeros beyond the used channels).  Features: channel
+ * 0 solid flag, 1 signed distance to the interface / ell (periodic jump
+ * flooding: + on fluid, - on solid), ell = the mean distance from a fluid
+ * node to the solid, returned.  Targets: the 10 moments above scaled by
+ * eta/(F ell^2) (velocity), 1/(8 F ell) (density), 1/(F ell) (stress).
+ * moments writes the same targets in FP32; inject builds populations from
+ * scaled moments m (device FP32 [n][channels >= 10]) in the given mode. */
+double slb_warm_features(LBSys lb, void *x_dev, int channels);
+void   slb_warm_targets(LBSys lb, void *t_dev, int channels, double ell);
+void   slb_warm_moments(LBSys lb, float *m_dev, int channels, double ell);
+void   slb_warm_inject(LBSys lb, const float *m_dev, int channels, int mode, double ell);
 /* iters Adam steps (cosine learning-rate decay from lr to lr/20) over
  * seeded epochs of the samples; loss = mean over the three target groups of
  * the relative mean-square error.  history (iters doubles) may be NULL.
@@ -228,13 +241,17 @@ void   slb_warm_set_mode(slb_warmnet *net, int mode);
 int    slb_warm_get_mode(const slb_warmnet *net);
 /* Initialise the populations from the prediction (the mode above). */
 void   slb_warm_start(slb_warmnet *net, LBSys lb);
-/* Cold-start fallback: after `probe` steps the warm residual must be below
- * probe_res, or the populations are reset to rest and the solve restarts.
- * note_cold records a cold solve's residual at the probe step and sets the
- * threshold to the median of the recorded values. */
+/* Cold-start fallback.  note_cold records a cold solve's residual history
+ * (res[k] after (k+1)*interval steps, n entries); the network keeps the
+ * median curve over the recorded solves.  A warm solve with the same
+ * interval falls back to a cold start (populations reset to rest) at the
+ * first check, from the second on, where the steps projected from its last
+ * residual decay ratio exceed the steps the median cold curve takes to the
+ * tolerance.  set_fallback replaces the curve by a single threshold
+ * probe_res checked from step `probe` on (probe 0: never fall back). */
 void   slb_warm_set_fallback(slb_warmnet *net, int probe, double probe_res);
 void   slb_warm_get_fallback(const slb_warmnet *net, int *probe, double *probe_res);
-void   slb_warm_note_cold(slb_warmnet *net, double res_at_probe);
+void   slb_warm_note_cold(slb_warmnet *net, const double *res, int n, int interval);
 /* Warm start, then run to slb_residual() < tol as slb_run_to_residual does,
  * with the fallback.  Returns the total steps; *used_warm is 0 when the
  * solve fell back to a cold start. */

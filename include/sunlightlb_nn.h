@@ -26,7 +26,11 @@ void slbn_set_stream(void *stream);
  * (BF16) and/or Yf (FP32) may be NULL. */
 void slbn_conv(const uint16_t *X, int cin, const uint16_t *W, const float *bias, uint16_t *Y,
                float *Yf, int cout, int relu, int transposed, int D, int H, int Wd);
-/* dW[co][t*cin+ci] += sum_n dY[n][co] X[n+off(t)][ci]; channels 16..64 step 16 */
+/* dW[co][t*cin+ci] += sum_n dY[n][co] X[n+off(t)][ci]; channels 16..64 step
+ * 16.  When slbn_wgrad_blocked_ok() (x extent a multiple of 8, channels 16,
+ * 32 or 64) it is a GEMM over a node-blocked copy of X and dY with a fixed
+ * summation order; otherwise a per-tap LDS-staged kernel with atomics. */
+int  slbn_wgrad_blocked_ok(int cin, int cout, int D, int H, int Wd);
 void slbn_conv_wgrad(const uint16_t *X, int cin, const uint16_t *dY, int cout, float *dW, int D,
                      int H, int Wd);
 /* the same for any channel counts that are multiples of 8 (scalar FP32) */

@@ -283,10 +283,12 @@ def jfa(seed_mask):
     return cur
 
 
-def features(solid, scale=0.125):
-    """(solid flag, signed distance * scale) per node as float32 before BF16
-    rounding: + distance to the nearest solid node on fluid nodes, - the
-    distance to the nearest fluid node on solid nodes (64 without any)."""
+def features(solid):
+    """(solid flag, signed distance / ell) per node as float32 before BF16
+    rounding, and ell: + distance to the nearest solid node on fluid nodes,
+    - the distance to the nearest fluid node on solid nodes (64 without
+    any); ell = the mean over fluid nodes of the distance to the solid
+    (float32 distances summed in float64; 1 without fluid or solid)."""
     shape = solid.shape
     idx = np.arange(solid.size)
     ns = jfa(solid)
@@ -296,8 +298,12 @@ def features(solid, scale=0.125):
                       np.float32(64.0))
     dist_f = np.where(nf >= 0, np.sqrt(_pdist2(shape, idx, np.maximum(nf, 0)).astype(np.float32)),
                       np.float32(64.0))
-    sd = np.where(s, -dist_f, dist_s).astype(np.float32) * np.float32(scale)
-    return np.stack([s.astype(np.float32), sd.astype(np.float32)], -1).reshape(shape + (2,))
+    fl = ~s
+    tot = dist_s[fl].astype(np.float64).sum()
+    ell = tot / fl.sum() if fl.any() and tot > 0 else 1.0
+    scale = np.float32(1.0 / ell)
+    sd = np.where(s, -dist_f, dist_s).astype(np.float32) * scale
+    return np.stack([s.astype(np.float32), sd.astype(np.float32)], -1).reshape(shape + (2,)), ell
 
 
 def exact_distance(seed_mask):

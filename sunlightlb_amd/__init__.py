@@ -156,6 +156,8 @@ SIGNATURES = {
     "slb_sum_pops": (None, [LBSys, RealP]),
     "slb_insert_spheres": (I, [LBSys, I, RealP]),
     "slb_rsa_pack": (I, [LBSys, D, D, C.c_ulonglong, C.c_long, RealP, I, RealP]),
+    "slb_pack_map": (I, [I, I, I, D, D, C.c_ulonglong, I, C.c_long, VP, RealP]),
+    "slb_insert_map": (LBObj, [LBSys, VP]),
     "slb_mp_set_adsorption": (None, [MPSys, D]), "slb_mp_track_moments": (None, [MPSys, I]),
     "slb_mp_sums": (None, [MPSys, RealP]), "slb_mp_run": (None, [MPSys, I]),
     "slb_mp_time_steps": (D, [MPSys, I]),
@@ -182,15 +184,15 @@ SIGNATURES = {
     "slb_warm_nsamples": (I, [C.c_void_p]),
     "slb_warm_sample_stats": (None, [C.c_void_p, I, RealP]),
     "slb_warm_set_mode": (None, [C.c_void_p, I]), "slb_warm_get_mode": (I, [C.c_void_p]),
-    "slb_warm_features": (None, [LBSys, VP, I]), "slb_warm_targets": (None, [LBSys, VP, I]),
-    "slb_warm_moments": (None, [LBSys, VP, I]), "slb_warm_inject": (None, [LBSys, VP, I, I]),
+    "slb_warm_features": (D, [LBSys, VP, I]), "slb_warm_targets": (None, [LBSys, VP, I, D]),
+    "slb_warm_moments": (None, [LBSys, VP, I, D]), "slb_warm_inject": (None, [LBSys, VP, I, I, D]),
     "slb_warm_train": (D, [C.c_void_p, I, D, C.c_ulonglong, RealP]),
     "slb_warm_eval": (D, [C.c_void_p, LBSys, RealP]),
     "slb_warm_predict": (None, [C.c_void_p, LBSys, VP]),
     "slb_warm_start": (None, [C.c_void_p, LBSys]),
     "slb_warm_set_fallback": (None, [C.c_void_p, I, D]),
     "slb_warm_get_fallback": (None, [C.c_void_p, IntP, RealP]),
-    "slb_warm_note_cold": (None, [C.c_void_p, D]),
+    "slb_warm_note_cold": (None, [C.c_void_p, RealP, I, I]),
     "slb_warm_solve": (I, [C.c_void_p, LBSys, D, I, I, RealP, IntP]),
     "slb_warm_nparams": (C.c_long, [C.c_void_p]),
     "slb_warm_layout": (None, [IntP, IntP, IntP, LongP, LongP]),
@@ -202,6 +204,7 @@ SIGNATURES = {
     "slbn_conv": (None, [VP, I, VP, VP, VP, VP, I, I, I, I, I, I]),
     "slbn_conv_wgrad": (None, [VP, I, VP, I, VP, I, I, I]),
     "slbn_conv_wgrad_small": (None, [VP, I, VP, I, VP, I, I, I]),
+    "slbn_wgrad_blocked_ok": (I, [I, I, I, I, I]),
     "slbn_wtrans": (None, [VP, I, I, VP]),
     "slbn_relu_bwd": (None, [VP, VP, VP, C.c_long]),
     "slbn_bias_grad": (None, [VP, I, I, VP]),

@@ -126,9 +126,10 @@ double slbd_copy_test(size_t bytes, int reps);
 /* ---- warm start (features, targets, injection) ---- */
 long slbd_nodes(const slbd_lattice *L);
 void slbd_dims(const slbd_lattice *L, int *d);
-/* channel 0 solid flag, channel 1 signed distance * scale (periodic jump
- * flooding); C-channel BF16 device tensor */
-void slbd_features(slbd_lattice *L, uint16_t *X, int C, float scale);
+/* channel 0 solid flag, channel 1 signed distance / ell (periodic jump
+ * flooding) with ell the mean distance from a fluid node to the nearest
+ * solid node (returned); C-channel BF16 device tensor */
+double slbd_features(slbd_lattice *L, uint16_t *X, int C);
 /* the 10 moments of the live state on fluid nodes (0 on solid), scaled per
  * group: 0..2 (j - b/2)/rho * sc[0], 3 drho * sc[1], 4..9 stress
  * (xx,yy,zz,xy,xz,yz) * sc[2]; BF16 [n][C] */

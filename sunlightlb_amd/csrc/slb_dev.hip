@@ -1494,6 +1494,27 @@ __global__ void k_jfa_pass(const int *__restrict__ in, int *__restrict__ out, in
   out[n] = best;
 }
 
+/* sum over fluid nodes of the distance to the nearest solid node (FP64,
+ * atomics), and the fluid node count */
+__global__ void __launch_bounds__(256)
+k_fluid_dist(const uint16_t *__restrict__ flags, const int *__restrict__ ns, double *__restrict__ out,
+             Geom G) {
+  const long n = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  double d = 0.0, c = 0.0;
+  if (n < G.N && !(flags[n] & FLAG_SOLID)) {
+    d = (ns[n] >= 0) ? (double)sqrtf((float)pdist2(G, n, ns[n])) : 64.0;
+    c = 1.0;
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    d += __shfl_xor(d, o, 64);
+    c += __shfl_xor(c, o, 64);
+  }
+  if ((threadIdx.x & 63) == 0 && c > 0.0) {
+    atomicAdd(out, d);
+    atomicAdd(out + 1, c);
+  }
+}
+
 /* channels: 0 solid (1/0), 1 signed distance * scale (+ to the nearest
  * solid node on fluid nodes, - to the nearest fluid node on solid nodes) */
 __global__ void k_features(const uint16_t *__restrict__ flags, const int *__restrict__ ns,
@@ -1617,7 +1638,7 @@ extern "C" long slbd_nodes(const slbd_lattice *L) { return L->N; }
 extern "C" void slbd_dims(const slbd_lattice *L, int *d) { d[0] = L->esi; d[1] = L->esj; d[2] = L->esk; }
 
 /* features of the lattice's solid map into a C-channel BF16 device tensor */
-extern "C" void slbd_features(slbd_lattice *L, uint16_t *X, int C, float scale) {
+extern "C" double slbd_features(slbd_lattice *L, uint16_t *X, int C) {
   int *buf[3];
   for (int b = 0; b < 3; b++) HIPCHECK(hipMalloc(&buf[b], sizeof(int) * L->N));
   int maxd = L->esi > L->esj ? L->esi : L->esj;
@@ -1641,11 +1662,23 @@ extern "C" void slbd_features(slbd_lattice *L, uint16_t *X, int C, float scale) 
     LAUNCHCHECK();
     res[phase] = pong;   /* ping is free for the next phase */
   }
+  /* normalising length: mean distance from a fluid node to the solid */
+  double *acc, h[2];
+  HIPCHECK(hipMalloc(&acc, 2 * sizeof(double)));
+  HIPCHECK(hipMemsetAsync(acc, 0, 2 * sizeof(double), L->stream));
+  hipLaunchKernelGGL(k_fluid_dist, lin_grid(L->N), dim3(256), 0, L->stream, L->flags, res[0], acc,
+                     L->G);
+  LAUNCHCHECK();
+  HIPCHECK(hipMemcpyAsync(h, acc, 2 * sizeof(double), hipMemcpyDeviceToHost, L->stream));
+  HIPCHECK(hipStreamSynchronize(L->stream));
+  HIPCHECK(hipFree(acc));
+  const double ell = (h[1] > 0.0 && h[0] > 0.0) ? h[0] / h[1] : 1.0;
   hipLaunchKernelGGL(k_features, lin_grid(L->N), dim3(256), 0, L->stream, L->flags, res[0], res[1],
-                     X, C, scale, L->G);
+                     X, C, (float)(1.0 / ell), L->G);
   LAUNCHCHECK();
   HIPCHECK(hipStreamSynchronize(L->stream));
   for (int b = 0; b < 3; b++) HIPCHECK(hipFree(buf[b]));
+  return ell;
 }
 
 template <typename O>

@@ -204,6 +204,140 @@ k_conv_wgrad(const uint16_t *__restrict__ X, int Cin, const uint16_t *__restrict
 }
 
 /* ---------------------------------------------------------------------- */
+/* Weight gradient as a GEMM over the node-blocked layout                    */
+/*   Xb[((n >> 3) * C + c) * 8 + (n & 7)]                                     */
+/* (8 consecutive nodes of one channel in 16 bytes, x-rows of a multiple of  */
+/* 8 nodes).  Block (group range, tap t): M = Cout, N = Cin, K = nodes; the   */
+/* 4 waves take interleaved K-steps of 32 nodes (4 groups) and are reduced in */
+/* LDS; partial sums per block go to part[t][block][co][ci] and are added in  */
+/* a fixed order by k_wgrad_reduce (deterministic).  The tap's x shift of one */
+/* node is a one-element funnel of two 8-node groups of the same x-row.        */
+
+__global__ void k_block(const uint16_t *__restrict__ X, int C, uint16_t *__restrict__ Xb, long N) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (N >> 3) * C) return;
+  const long g = e / C;
+  const int c = (int)(e - g * C);
+  uint32_t w[4];
+#pragma unroll
+  for (int k = 0; k < 4; k++)
+    w[k] = (uint32_t)X[(g * 8 + 2 * k) * C + c] | ((uint32_t)X[(g * 8 + 2 * k + 1) * C + c] << 16);
+  *(uint4 *)(Xb + e * 8) = make_uint4(w[0], w[1], w[2], w[3]);
+}
+
+typedef __attribute__((ext_vector_type(4))) uint32_t u32x4;
+
+__device__ __forceinline__ bf16x8 as_bf(u32x4 v) { return __builtin_bit_cast(bf16x8, v); }
+
+template <int MT, int NT>
+__global__ void __launch_bounds__(256)
+k_wgrad_b(const uint16_t *__restrict__ Xb, const uint16_t *__restrict__ Db,
+          float *__restrict__ part, int gpb, NGeom G) {
+  constexpr int Cin = 16 * NT, Cout = 16 * MT;
+  __shared__ float red[3][MT * NT * 4][64];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, grp = lane >> 4, m = lane & 15;
+  const int t = blockIdx.y;
+  const int dz = t / 9 - 1, dy = (t / 3) % 3 - 1, dx = t % 3 - 1;
+  const int gW = G.Wd >> 3;
+  const long ngr = (long)G.N >> 3;
+  const long gb = (long)blockIdx.x * gpb;
+  const long ge = min(ngr, gb + gpb);
+  f32x4 acc[MT][NT];
+#pragma unroll
+  for (int i = 0; i < MT; i++)
+#pragma unroll
+    for (int j = 0; j < NT; j++) acc[i][j] = (f32x4){0.f, 0.f, 0.f, 0.f};
+  for (long g0 = gb + 4 * wv; g0 < ge; g0 += 16) {
+    const long g = g0 + grp;
+    const bool ok = g < ge;
+    const long gg = ok ? g : gb;
+    bf16x8 a[MT], b[NT];
+#pragma unroll
+    for (int i = 0; i < MT; i++) {
+      u32x4 v = *(const u32x4 *)(Db + (gg * Cout + 16 * i + m) * 8);
+      if (!ok) v = (u32x4){0u, 0u, 0u, 0u};
+      a[i] = as_bf(v);
+    }
+    const int row = (int)(gg / gW), xg = (int)(gg - (long)row * gW);
+    const int z = row / G.H, y = row - z * G.H;
+    int zz = z + dz, yy = y + dy;
+    zz += (zz < 0) ? G.D : 0; zz -= (zz >= G.D) ? G.D : 0;
+    yy += (yy < 0) ? G.H : 0; yy -= (yy >= G.H) ? G.H : 0;
+    const long rb = ((long)zz * G.H + yy) * gW;
+    const long s0 = rb + xg;
+    int xn = xg + dx;
+    xn += (xn < 0) ? gW : 0; xn -= (xn >= gW) ? gW : 0;
+    const long s1 = rb + xn;
+#pragma unroll
+    for (int j = 0; j < NT; j++) {
+      const int ci = 16 * j + m;
+      u32x4 v0 = *(const u32x4 *)(Xb + (s0 * Cin + ci) * 8);
+      u32x4 v;
+      if (dx == 0) {
+        v = v0;
+      } else {
+        const u32x4 v1 = *(const u32x4 *)(Xb + (s1 * Cin + ci) * 8);
+        if (dx > 0) {
+          /* nodes x0+1 .. x0+8: elements 1..7 of v0, element 0 of v1 */
+          v.x = (v0.x >> 16) | (v0.y << 16);
+          v.y = (v0.y >> 16) | (v0.z << 16);
+          v.z = (v0.z >> 16) | (v0.w << 16);
+          v.w = (v0.w >> 16) | (v1.x << 16);
+        } else {
+          /* nodes x0-1 .. x0+6: element 7 of v1, elements 0..6 of v0 */
+          v.x = (v1.w >> 16) | (v0.x << 16);
+          v.y = (v0.x >> 16) | (v0.y << 16);
+          v.z = (v0.y >> 16) | (v0.z << 16);
+          v.w = (v0.z >> 16) | (v0.w << 16);
+        }
+      }
+      if (!ok) v = (u32x4){0u, 0u, 0u, 0u};
+      b[j] = as_bf(v);
+    }
+#pragma unroll
+    for (int i = 0; i < MT; i++)
+#pragma unroll
+      for (int j = 0; j < NT; j++)
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+  }
+  /* reduce the 4 waves' partial tiles: waves 1..3 write, wave 0 adds */
+  if (wv > 0) {
+#pragma unroll
+    for (int i = 0; i < MT; i++)
+#pragma unroll
+      for (int j = 0; j < NT; j++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) red[wv - 1][(i * NT + j) * 4 + r][lane] = acc[i][j][r];
+  }
+  __syncthreads();
+  if (wv > 0) return;
+  float *out = part + ((long)t * gridDim.x + blockIdx.x) * Cout * Cin;
+#pragma unroll
+  for (int i = 0; i < MT; i++)
+#pragma unroll
+    for (int j = 0; j < NT; j++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int q = (i * NT + j) * 4 + r;
+        const float v = acc[i][j][r] + red[0][q][lane] + red[1][q][lane] + red[2][q][lane];
+        /* lane holds D[co = 16i + 4 grp + r][ci = 16j + m] */
+        out[(16 * i + 4 * grp + r) * Cin + 16 * j + m] = v;
+      }
+}
+
+/* dW[co][t*Cin + ci] += sum over blocks b (in order) of part[t][b][co][ci] */
+__global__ void k_wgrad_reduce(const float *__restrict__ part, int nb, int Cout, int Cin, int Kp,
+                               float *__restrict__ dW) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= 27 * Cout * Cin) return;
+  const int t = e / (Cout * Cin), rem = e - t * Cout * Cin, co = rem / Cin, ci = rem - co * Cin;
+  const float *p = part + (long)t * nb * Cout * Cin + rem;
+  float s = 0.f;
+  for (int b = 0; b < nb; b++) s += p[(long)b * Cout * Cin];
+  dW[(long)co * Kp + t * Cin + ci] += s;
+}
+
+/* ---------------------------------------------------------------------- */
 /* Elementwise and reduction helpers.                                        */
 
 /* WT[ci][t*Cout + co] = W[co][t*Cin + ci]  (taps of the transposed conv) */
@@ -385,15 +519,24 @@ extern "C" void slbn_conv(const uint16_t *X, int cin, const uint16_t *W, const f
   LAUNCHCHECK();
 }
 
-extern "C" void slbn_conv_wgrad(const uint16_t *X, int cin, const uint16_t *dY, int cout,
-                                float *dW, int D, int H, int Wd) {
-  const NGeom G = ngeom(D, H, Wd);
-  const int Kp = slbn_kpad(cin);
-  if (cin > 64 || cout > 64 || (cin % 16) || (cout % 16)) {
-    fprintf(stderr, "slbn_conv_wgrad: channels must be 16..64 in steps of 16\n");
-    exit(EXIT_FAILURE);
+/* workspace of the blocked weight gradient (grown on demand, kept) */
+static void *wg_ws = nullptr;
+static size_t wg_ws_bytes = 0;
+static void *wg_workspace(size_t bytes) {
+  if (bytes > wg_ws_bytes) {
+    HIPCHECK(hipStreamSynchronize(nn_stream));
+    if (wg_ws) HIPCHECK(hipFree(wg_ws));
+    HIPCHECK(hipMalloc(&wg_ws, bytes));
+    wg_ws_bytes = bytes;
   }
-  /* about 256 node chunks per block keeps the register accumulation long */
+  return wg_ws;
+}
+
+/* the original per-tap LDS-staged kernel: any extents (used when the
+ * x-rows are not a multiple of 8 nodes) */
+static void wgrad_staged(const uint16_t *X, int cin, const uint16_t *dY, int cout, float *dW,
+                         const NGeom &G) {
+  const int Kp = slbn_kpad(cin);
   int npb = WG_CHUNK * 64;
   if ((long)npb * 64 < G.N) npb = (int)(((G.N + 63) / 64 + WG_CHUNK - 1) / WG_CHUNK) * WG_CHUNK;
   const unsigned g = blocks_for(G.N, npb);
@@ -409,6 +552,63 @@ extern "C" void slbn_conv_wgrad(const uint16_t *X, int cin, const uint16_t *dY, 
 #undef SLBN_WG
     LAUNCHCHECK();
   }
+}
+
+extern "C" int slbn_wgrad_blocked_ok(int cin, int cout, int D, int H, int Wd) {
+  (void)D; (void)H;
+  return Wd % 8 == 0 && (cin == 16 || cin == 32 || cin == 64) &&
+         (cout == 16 || cout == 32 || cout == 64);
+}
+
+extern "C" void slbn_conv_wgrad(const uint16_t *X, int cin, const uint16_t *dY, int cout,
+                                float *dW, int D, int H, int Wd) {
+  const NGeom G = ngeom(D, H, Wd);
+  const int Kp = slbn_kpad(cin);
+  if (cin > 64 || cout > 64 || (cin % 16) || (cout % 16)) {
+    fprintf(stderr, "slbn_conv_wgrad: channels must be 16..64 in steps of 16\n");
+    exit(EXIT_FAILURE);
+  }
+  if (!slbn_wgrad_blocked_ok(cin, cout, D, H, Wd)) {
+    wgrad_staged(X, cin, dY, cout, dW, G);
+    return;
+  }
+  const long ngr = (long)G.N >> 3;
+  /* enough blocks per tap to fill the chip with 27 taps, each block a
+   * contiguous group range in steps of 16 groups */
+  long nb = (ngr + 255) / 256;
+  if (nb > 64) nb = 64;
+  if (nb < 1) nb = 1;
+  int gpb = (int)(((ngr + nb - 1) / nb + 15) / 16 * 16);
+  nb = (ngr + gpb - 1) / gpb;
+  const size_t xb = (size_t)G.N * cin * 2, db = (size_t)G.N * cout * 2;
+  const size_t pb = (size_t)27 * nb * cout * cin * 4;
+  char *ws = (char *)wg_workspace(xb + db + pb);
+  uint16_t *Xb = (uint16_t *)ws, *Db = (uint16_t *)(ws + xb);
+  float *part = (float *)(ws + xb + db);
+  hipLaunchKernelGGL(k_block, dim3(blocks_for(ngr * cin, 256)), dim3(256), 0, nn_stream, X, cin,
+                     Xb, (long)G.N);
+  LAUNCHCHECK();
+  hipLaunchKernelGGL(k_block, dim3(blocks_for(ngr * cout, 256)), dim3(256), 0, nn_stream, dY, cout,
+                     Db, (long)G.N);
+  LAUNCHCHECK();
+  const dim3 grid((unsigned)nb, 27);
+#define SLBN_WGB(MT, NT) \
+  hipLaunchKernelGGL((k_wgrad_b<MT, NT>), grid, dim3(256), 0, nn_stream, Xb, Db, part, gpb, G)
+  const int mt = cout / 16, nt = cin / 16;
+  if (mt == 1 && nt == 1) SLBN_WGB(1, 1);
+  else if (mt == 1 && nt == 2) SLBN_WGB(1, 2);
+  else if (mt == 1 && nt == 4) SLBN_WGB(1, 4);
+  else if (mt == 2 && nt == 1) SLBN_WGB(2, 1);
+  else if (mt == 2 && nt == 2) SLBN_WGB(2, 2);
+  else if (mt == 2 && nt == 4) SLBN_WGB(2, 4);
+  else if (mt == 4 && nt == 1) SLBN_WGB(4, 1);
+  else if (mt == 4 && nt == 2) SLBN_WGB(4, 2);
+  else SLBN_WGB(4, 4);
+#undef SLBN_WGB
+  LAUNCHCHECK();
+  hipLaunchKernelGGL(k_wgrad_reduce, dim3(blocks_for(27L * cout * cin, 256)), dim3(256), 0,
+                     nn_stream, part, (int)nb, cout, cin, Kp, dW);
+  LAUNCHCHECK();
 }
 
 extern "C" void slbn_wtrans(const uint16_t *W, int cout, int cin, uint16_t *WT) {

@@ -626,3 +626,85 @@ int slb_rsa_pack(LBSys lb, double R, double phi, unsigned long long seed, long m
   free(g.head); free(g.next); free(g.xyz);
   return placed;
 }
+
+/* ------------------------------------------------------------------ */
+/* Solid maps of sphere packs without a lattice (thread-safe: no globals,
+ * no objects), and their insertion as one object.                       */
+
+/* Mark the nodes of one sphere image; returns the nodes that were fluid. */
+static long raster_map(unsigned char *solid, const int *L, double R, const double *o) {
+  int lo[3], hi[3];
+  for (int d = 0; d < 3; d++) axis_range(o[d], R, L[d], &lo[d], &hi[d]);
+  const double radsq = R * R;
+  long added = 0;
+  for (int i = lo[0]; i <= hi[0]; i++)
+    for (int j = lo[1]; j <= hi[1]; j++)
+      for (int k = lo[2]; k <= hi[2]; k++)
+        if (in_sphere(i, j, k, o[0], o[1], o[2], radsq)) {
+          long n = (long)L[2] * ((long)L[1] * i + j) + k;
+          if (!solid[n]) { solid[n] = 1; added++; }
+        }
+  return added;
+}
+
+int slb_pack_map(int esi, int esj, int esk, double R, double phi, unsigned long long seed,
+                 int overlap, long maxtrials, unsigned char *solid, double *phi_out) {
+  const int L[3] = {esi, esj, esk};
+  const long N = (long)esi * esj * esk;
+  rsa_grid g;
+  memset(&g, 0, sizeof g);
+  int ncell = 1;
+  for (int d = 0; d < 3; d++) {
+    g.L[d] = L[d];
+    g.nc[d] = (int)floor(g.L[d] / (2.0 * R));
+    if (g.nc[d] < 3) g.nc[d] = 1;
+    g.cs[d] = g.L[d] / g.nc[d];
+    ncell *= g.nc[d];
+  }
+  if (!overlap) {
+    g.head = (int *)malloc(sizeof(int) * ncell);
+    if (!g.head) return -1;
+    for (int c = 0; c < ncell; c++) g.head[c] = -1;
+  }
+  memset(solid, 0, (size_t)N);
+  uint64_t st = (uint64_t)seed;
+  const long target = (long)ceil(phi * (double)N);
+  long nsolid = 0;
+  int placed = 0;
+  for (long t = 0; t < maxtrials && nsolid < target; t++) {
+    double c[3] = {unit01(&st) * g.L[0], unit01(&st) * g.L[1], unit01(&st) * g.L[2]};
+    if (!overlap) {
+      if (rsa_overlaps(&g, c, 4.0 * R * R)) continue;
+      rsa_add(&g, c);
+    }
+    for (int a = -1; a <= 1; a++)
+      for (int b = -1; b <= 1; b++)
+        for (int e = -1; e <= 1; e++) {
+          double o[3] = {c[0] + a * g.L[0], c[1] + b * g.L[1], c[2] + e * g.L[2]};
+          if (o[0] + R < 0.0 || o[0] - R > g.L[0] || o[1] + R < 0.0 || o[1] - R > g.L[1] ||
+              o[2] + R < 0.0 || o[2] - R > g.L[2])
+            continue;
+          nsolid += raster_map(solid, L, R, o);
+        }
+    placed++;
+  }
+  free(g.head); free(g.next); free(g.xyz);
+  if (phi_out) *phi_out = (double)nsolid / (double)N;
+  return placed;
+}
+
+LBObj slb_insert_map(LBSys lb, const unsigned char *solid) {
+  LBObj p = addobj(lb);
+  zeroobj(p);
+  long ns = 0;
+  for (long n = 0; n < lb->nnodes; n++)
+    if (solid[n]) { lb->obj[n] = p; ns++; }
+  if (ns == 0) {
+    drop_head(lb, p);
+    lb->next_id--;
+    return NULL;
+  }
+  lb->changed_objects = 1;
+  slb_obj_changed(lb);
+  return p;
+}

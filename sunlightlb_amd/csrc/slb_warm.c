@@ -47,12 +47,15 @@ static const int LEVEL_CMAX[NLEVEL] = {16, 32, 64, 64};
 #define WN_C 16          /* channels of the input and output tensors */
 #define WN_FEAT 2        /* used input channels: solid flag, signed distance */
 #define WN_TARG 10       /* used output channels: u (3), drho, stress (6) */
-#define WN_SDF_SCALE 0.125f
-/* target group scales: u eta/F * WN_SU, drho/F * WN_SR, stress/F * WN_SS */
-#define WN_SU 0.25
-#define WN_SR 0.03125
+/* Targets per group, with ell the pack's normalising length (mean distance
+ * from a fluid node to the solid) -- a Stokes solution at fixed F and eta
+ * scales as ell^2 (velocity), ell (density deviation, stress):
+ *   u eta / (F ell^2) * WN_SU,  drho / (F ell) * WN_SR,  s / (F ell) * WN_SS */
+#define WN_SU 1.0
+#define WN_SR 0.125
 #define WN_SS 1.0
-#define WN_MAGIC "SLBWNET2"
+#define WN_MAGIC "SLBWNET3"
+#define WN_CURVE 256     /* checks of the cold reference curve */
 static const int WN_GROUP[WN_TARG] = {0, 0, 0, 1, 2, 2, 2, 2, 2, 2};
 
 typedef struct {
@@ -62,6 +65,7 @@ typedef struct {
   uint16_t *t;   /* [n][WN_TARG] */
   float *wc;     /* device: per-channel loss weights (WN_TARG) */
   double ms[3];  /* mean square of each target group */
+  double ell;    /* normalising length */
 } wn_sample;
 
 struct slb_warmnet {
@@ -86,8 +90,10 @@ struct slb_warmnet {
   /* cold-start fallback */
   int probe;
   double probe_res;
-  int nnote;
+  int nnote, note_interval;
   double *notes;
+  int ncurve, curve_interval;
+  double curve[WN_CURVE];
   uint64_t rng;
 };
 
@@ -358,23 +364,25 @@ static void backward(slb_warmnet *net) {
 /* ------------------------------------------------------------------ */
 /* Lattice side.                                                         */
 
-/* group scales of the targets for this lattice (per unit force) */
-static void target_scales(LBSys lb, const slbd_params *P, float sc[3]) {
+/* group scales of the targets for this lattice (per unit force and
+ * normalising length ell) */
+static void target_scales(LBSys lb, const slbd_params *P, double ell, float sc[3]) {
   const double F = sqrt(P->bf[0] * P->bf[0] + P->bf[1] * P->bf[1] + P->bf[2] * P->bf[2]);
   const double eta = lb->rho0 * (lb->tau - 0.5) / 3.0;
   if (!(F > 0.0) || P->bf_mode == 2) die("the warm start needs a uniform, non-zero body force");
-  sc[0] = (float)(eta / F * WN_SU);
-  sc[1] = (float)(WN_SR / F);
-  sc[2] = (float)(WN_SS / F);
+  sc[0] = (float)(eta / (F * ell * ell) * WN_SU);
+  sc[1] = (float)(WN_SR / (F * ell));
+  sc[2] = (float)(WN_SS / (F * ell));
 }
 
-static void lattice_features(slb_warmnet *net, LBSys lb, slbd_params *P) {
+/* features into net->x0; returns ell */
+static double lattice_features(slb_warmnet *net, LBSys lb, slbd_params *P) {
   slb_params(lb, P);
   slb_dev_geom(lb);
   int d[3] = {lb->esi, lb->esj, lb->esk};
   ensure_acts(net, d);
   slbn_sync();
-  slbd_features(PRIV(lb)->dev, net->x0, WN_C, WN_SDF_SCALE);
+  return slbd_features(PRIV(lb)->dev, net->x0, WN_C);
 }
 
 /* Mean square of each target group (host pass over a device tensor) and
@@ -422,18 +430,19 @@ int slb_warm_add_sample(slb_warmnet *net, LBSys lb) {
   s->t = slbn_malloc(N * WN_TARG * 2);
   s->wc = slbn_malloc(4 * WN_TARG);
   float sc[3], wc[WN_TARG];
-  target_scales(lb, &P, sc);
   slbn_sync();
-  slbd_features(PRIV(lb)->dev, s->x, WN_FEAT, WN_SDF_SCALE);
+  s->ell = slbd_features(PRIV(lb)->dev, s->x, WN_FEAT);
+  target_scales(lb, &P, s->ell, sc);
   slbd_targets(PRIV(lb)->dev, &P, sc, s->t, WN_TARG);
   target_weights(s->t, N, s->ms, wc);
   slbn_h2d(s->wc, wc, sizeof(wc));
   return ++net->nsamp;
 }
 
-void slb_warm_sample_stats(const slb_warmnet *net, int i, double ms[3]) {
+void slb_warm_sample_stats(const slb_warmnet *net, int i, double out[4]) {
   if (i < 0 || i >= net->nsamp) die("sample index out of range");
-  memcpy(ms, net->samp[i].ms, sizeof(double) * 3);
+  memcpy(out, net->samp[i].ms, sizeof(double) * 3);
+  out[3] = net->samp[i].ell;
 }
 
 int slb_warm_nsamples(const slb_warmnet *net) { return net->nsamp; }
@@ -454,44 +463,44 @@ static void inject(LBSys lb, const slbd_params *P, const float sc[3], const floa
   }
 }
 
-void slb_warm_features(LBSys lb, void *x, int channels) {
+double slb_warm_features(LBSys lb, void *x, int channels) {
   slbd_params P;
   slb_params(lb, &P);
   slb_dev_geom(lb);
   slbn_sync();
-  slbd_features(PRIV(lb)->dev, (uint16_t *)x, channels, WN_SDF_SCALE);
+  return slbd_features(PRIV(lb)->dev, (uint16_t *)x, channels);
 }
 
-void slb_warm_moments(LBSys lb, float *m, int channels) {
+void slb_warm_moments(LBSys lb, float *m, int channels, double ell) {
   slbd_params P;
   float sc[3];
   slb_params(lb, &P);
   slb_dev_geom(lb);
   slb_dev_pops(lb);
-  target_scales(lb, &P, sc);
+  target_scales(lb, &P, ell, sc);
   slbn_sync();
   slbd_moments(PRIV(lb)->dev, &P, sc, m, channels);
 }
 
-void slb_warm_inject(LBSys lb, const float *m, int channels, int mode) {
+void slb_warm_inject(LBSys lb, const float *m, int channels, int mode, double ell) {
   slbd_params P;
   float sc[3];
   if (channels < WN_TARG) die("slb_warm_inject needs at least 10 channels");
   if (mode != SLB_WARM_EQUILIBRIUM && mode != SLB_WARM_MOMENTS) die("unknown warm-start mode");
   slb_params(lb, &P);
   slb_dev_geom(lb);
-  target_scales(lb, &P, sc);
+  target_scales(lb, &P, ell, sc);
   slbn_sync();
   inject(lb, &P, sc, m, channels, mode);
 }
 
-void slb_warm_targets(LBSys lb, void *t, int channels) {
+void slb_warm_targets(LBSys lb, void *t, int channels, double ell) {
   slbd_params P;
   float sc[3];
   slb_params(lb, &P);
   slb_dev_geom(lb);
   slb_dev_pops(lb);
-  target_scales(lb, &P, sc);
+  target_scales(lb, &P, ell, sc);
   slbn_sync();
   slbd_targets(PRIV(lb)->dev, &P, sc, (uint16_t *)t, channels);
 }
@@ -534,8 +543,11 @@ double slb_warm_train(slb_warmnet *net, int iters, double lr, unsigned long long
         pos = 0;
       }
       const int it = it0 + b;
-      /* cosine decay to lr/20 */
-      const double f = 0.05 + 0.95 * 0.5 * (1.0 + cos(3.141592653589793 * it / (double)iters));
+      /* linear warm-up over the first 5% of the steps, then cosine decay
+       * to lr/20 */
+      const double wu = 0.05 * iters;
+      double f = 0.05 + 0.95 * 0.5 * (1.0 + cos(3.141592653589793 * it / (double)iters));
+      if (it < wu) f *= (it + 1) / wu;
       train_step(net, &net->samp[perm[pos++]], lr * f, b);
     }
     slbn_d2h(hl, net->loss, 4 * nb);
@@ -558,8 +570,8 @@ double slb_warm_eval(slb_warmnet *net, LBSys lb, double *grp) {
   float sc[3], wc[WN_TARG];
   double ms[3];
   slb_dev_pops(lb);
-  lattice_features(net, lb, &P);
-  target_scales(lb, &P, sc);
+  const double ell = lattice_features(net, lb, &P);
+  target_scales(lb, &P, ell, sc);
   forward(net);
   slbn_sync();
   const long N = net->nl[0];
@@ -590,8 +602,8 @@ int slb_warm_get_mode(const slb_warmnet *net) { return net->mode; }
 void slb_warm_start(slb_warmnet *net, LBSys lb) {
   slbd_params P;
   float sc[3];
-  lattice_features(net, lb, &P);
-  target_scales(lb, &P, sc);
+  const double ell = lattice_features(net, lb, &P);
+  target_scales(lb, &P, ell, sc);
   forward(net);
   slbn_sync();
   inject(lb, &P, sc, net->outf, WN_C, net->mode);
@@ -600,8 +612,8 @@ void slb_warm_start(slb_warmnet *net, LBSys lb) {
 void slb_warm_predict(slb_warmnet *net, LBSys lb, float *m) {
   slbd_params P;
   float sc[3];
-  lattice_features(net, lb, &P);
-  target_scales(lb, &P, sc);
+  const double ell = lattice_features(net, lb, &P);
+  target_scales(lb, &P, ell, sc);
   forward(net);
   const long N = net->nl[0];
   float *h = malloc(N * WN_C * 4);
@@ -619,21 +631,7 @@ void slb_warm_predict(slb_warmnet *net, LBSys lb, float *m) {
 void slb_warm_set_fallback(slb_warmnet *net, int probe, double probe_res) {
   net->probe = probe;
   net->probe_res = probe_res;
-}
-
-void slb_warm_note_cold(slb_warmnet *net, double res_at_probe) {
-  double *n = realloc(net->notes, sizeof(double) * (net->nnote + 1));
-  if (!n) die("out of host memory");
-  net->notes = n;
-  net->notes[net->nnote++] = res_at_probe;
-  /* threshold: the median cold residual at the probe step */
-  double *t = malloc(sizeof(double) * net->nnote);
-  if (!t) die("out of host memory");
-  memcpy(t, net->notes, sizeof(double) * net->nnote);
-  for (int i = 1; i < net->nnote; i++)
-    for (int j = i; j > 0 && t[j - 1] > t[j]; j--) { const double x = t[j]; t[j] = t[j - 1]; t[j - 1] = x; }
-  net->probe_res = t[net->nnote / 2];
-  free(t);
+  net->ncurve = 0;
 }
 
 void slb_warm_get_fallback(const slb_warmnet *net, int *probe, double *probe_res) {
@@ -641,41 +639,91 @@ void slb_warm_get_fallback(const slb_warmnet *net, int *probe, double *probe_res
   if (probe_res) *probe_res = net->probe_res;
 }
 
-static int run_chunks(LBSys lb, double tol, int interval, int maxsteps, int stop_at, double *res,
-                      int *steps) {
-  /* runs from *steps until the residual is below tol (returns 1), maxsteps
-   * is reached, or *steps reaches stop_at (returns 0) */
-  while (*steps < maxsteps && *steps < stop_at) {
-    int n = interval;
-    if (*steps + n > maxsteps) n = maxsteps - *steps;
-    slb_run(lb, n);
-    *steps += n;
-    *res = slb_residual(lb);
-    if (*res >= 0.0 && *res < tol) return 1;
+/* Cold reference curve: the median over the recorded cold solves of the
+ * residual after k * interval steps (a finished solve counts as its last
+ * residual from then on). */
+static double median(double *v, int n) {
+  for (int i = 1; i < n; i++)
+    for (int j = i; j > 0 && v[j - 1] > v[j]; j--) { const double x = v[j]; v[j] = v[j - 1]; v[j - 1] = x; }
+  return v[n / 2];
+}
+
+void slb_warm_note_cold(slb_warmnet *net, const double *res, int n, int interval) {
+  if (n < 1) return;
+  if (net->nnote == 0 || interval != net->note_interval) {
+    net->nnote = 0;
+    net->note_interval = interval;
   }
-  return 0;
+  double *nn = realloc(net->notes, sizeof(double) * (size_t)(net->nnote + 1) * WN_CURVE);
+  if (!nn) die("out of host memory");
+  net->notes = nn;
+  double *row = net->notes + (size_t)net->nnote * WN_CURVE;
+  for (int k = 0; k < WN_CURVE; k++) row[k] = res[k < n ? k : n - 1];
+  net->nnote++;
+  double *t = malloc(sizeof(double) * net->nnote);
+  if (!t) die("out of host memory");
+  for (int k = 0; k < WN_CURVE; k++) {
+    for (int i = 0; i < net->nnote; i++) t[i] = net->notes[(size_t)i * WN_CURVE + k];
+    net->curve[k] = median(t, net->nnote);
+  }
+  free(t);
+  net->ncurve = WN_CURVE;
+  net->curve_interval = interval;
+  net->probe = 2 * interval;
+  net->probe_res = net->curve[1];
 }
 
 int slb_warm_solve(slb_warmnet *net, LBSys lb, double tol, int interval, int maxsteps, double *res,
                    int *used_warm) {
   if (interval < 1) interval = 1;
   double r = -1.0;
-  int steps = 0, warm = 1;
+  int steps = 0, warm = 1, done = 0;
   slb_warm_start(net, lb);
   slb_residual_reset(lb);
   slb_residual(lb);
-  int done = run_chunks(lb, tol, interval, maxsteps, net->probe > 0 ? net->probe : maxsteps, &r,
-                        &steps);
-  if (!done && steps < maxsteps && net->probe > 0 && !(r >= 0.0 && r < net->probe_res)) {
-    /* not falling faster than a cold start: restart from rest */
-    warm = 0;
-    slb_lbpriv *S = PRIV(lb);
-    slbd_zero_pops(S->dev);
-    S->pops_loc = LOC_DEV;
-    slb_residual_reset(lb);
-    slb_residual(lb);
+  /* cold steps to tol on the reference curve */
+  const int use_curve = net->ncurve > 0 && interval == net->curve_interval;
+  double cold_steps = HUGE_VAL;
+  if (use_curve) {
+    int k = 0;
+    while (k < net->ncurve && !(net->curve[k] < tol)) k++;
+    cold_steps = (double)(k + 1) * interval;
   }
-  if (!done) run_chunks(lb, tol, interval, maxsteps, maxsteps, &r, &steps);
+  double rprev = -1.0;
+  while (steps < maxsteps) {
+    int n = interval;
+    if (steps + n > maxsteps) n = maxsteps - steps;
+    slb_run(lb, n);
+    steps += n;
+    r = slb_residual(lb);
+    if (r >= 0.0 && r < tol) { done = 1; break; }
+    if (!warm) continue;
+    /* The warm residual must fall faster than a cold start: with the
+     * recorded curve, from the second check on, the steps projected from
+     * the last decay ratio must not exceed the cold steps to tol; without
+     * it, the residual must be below probe_res from step `probe` on. */
+    int fall_back = 0;
+    if (use_curve) {
+      if (rprev > 0.0 && steps >= 2 * interval) {
+        const double q = r / rprev;
+        const double proj = q < 1.0 ? steps + interval * log(tol / r) / log(q) : HUGE_VAL;
+        fall_back = proj > cold_steps;
+      }
+    } else if (net->probe > 0 && steps >= net->probe) {
+      fall_back = !(r >= 0.0 && r < net->probe_res);
+    }
+    rprev = r;
+    if (fall_back) {
+      /* not falling faster than a cold start: restart from rest */
+      warm = 0;
+      slb_lbpriv *S = PRIV(lb);
+      slbd_zero_pops(S->dev);
+      S->pops_loc = LOC_DEV;
+      slb_residual_reset(lb);
+      slb_residual(lb);
+    }
+  }
+  (void)done;
   if (res) *res = r;
   if (used_warm) *used_warm = warm;
   return steps;
@@ -696,6 +744,9 @@ int slb_warm_save(const slb_warmnet *net, const char *path) {
   hdr[2 * NLAYER + 1] = net->probe;
   int ok = fwrite(WN_MAGIC, 1, 8, f) == 8 && fwrite(hdr, sizeof(hdr), 1, f) == 1 &&
            fwrite(&net->probe_res, sizeof(double), 1, f) == 1 &&
+           fwrite(&net->ncurve, sizeof(int), 1, f) == 1 &&
+           fwrite(&net->curve_interval, sizeof(int), 1, f) == 1 &&
+           fwrite(net->curve, sizeof(double), WN_CURVE, f) == WN_CURVE &&
            fwrite(&net->nparam, sizeof(long), 1, f) == 1 &&
            fwrite(hw, sizeof(float), net->nparam, f) == (size_t)net->nparam;
   free(hw);
@@ -713,10 +764,14 @@ slb_warmnet *slb_warm_load(const char *path) {
   slb_warmnet tmp;
   memset(&tmp, 0, sizeof(tmp));
   layout(&tmp);
+  int ncurve = 0, cint = 0;
+  double curve[WN_CURVE];
   int ok = fread(magic, 1, 8, f) == 8 && memcmp(magic, WN_MAGIC, 8) == 0 &&
            fread(hdr, sizeof(hdr), 1, f) == 1 && hdr[0] == NLAYER &&
-           fread(&pr, sizeof(double), 1, f) == 1 && fread(&np, sizeof(long), 1, f) == 1 &&
-           np == tmp.nparam;
+           fread(&pr, sizeof(double), 1, f) == 1 && fread(&ncurve, sizeof(int), 1, f) == 1 &&
+           fread(&cint, sizeof(int), 1, f) == 1 &&
+           fread(curve, sizeof(double), WN_CURVE, f) == WN_CURVE &&
+           fread(&np, sizeof(long), 1, f) == 1 && np == tmp.nparam;
   for (int L = 0; ok && L < NLAYER; L++)
     if (hdr[1 + 2 * L] != LCIN[L] || hdr[2 + 2 * L] != LCOUT[L]) ok = 0;
   float *hw = ok ? malloc(sizeof(float) * np) : NULL;
@@ -730,6 +785,9 @@ slb_warmnet *slb_warm_load(const char *path) {
   free(hw);
   net->probe = hdr[2 * NLAYER + 1];
   net->probe_res = pr;
+  net->ncurve = ncurve;
+  net->curve_interval = cint;
+  memcpy(net->curve, curve, sizeof(curve));
   return net;
 }
 

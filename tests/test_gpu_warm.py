@@ -139,10 +139,12 @@ def test_conv_input_gradient_matches_oracle(shape, cin, cout):
 
 @pytest.mark.parametrize("shape,cin,cout,small", [((16, 16, 16), 16, 16, 0), ((8, 16, 24), 32, 64, 0),
                                                   ((32, 32, 32), 64, 64, 0), ((16, 8, 8), 64, 32, 0),
+                                                  ((8, 8, 12), 16, 32, 0), ((4, 6, 6), 64, 16, 0),
                                                   ((8, 8, 12), 8, 16, 1), ((6, 8, 10), 16, 8, 1)])
 def test_conv_weight_gradient_matches_oracle(shape, cin, cout, small):
-    print(f"conv wgrad {shape} {cin}->{cout} small={small}", flush=True)
     L = lib()
+    print(f"conv wgrad {shape} {cin}->{cout} small={small} "
+          f"blocked={L.slbn_wgrad_blocked_ok(cin, cout, *shape)}", flush=True)
     rng = np.random.default_rng(30 + cin + cout)
     xb, x = bf_random(rng, shape + (cin,))
     gb, g = bf_random(rng, shape + (cout,))
@@ -335,13 +337,14 @@ def test_features_match_oracle_jump_flooding():
     lat = pack_lattice((16, 16, 24), SPHERES)
     N = lat.N
     dF = Dev(N * 2 * 2)
-    L.slb_warm_features(lat.h, dF.p, 2)
+    ell = L.slb_warm_features(lat.h, dF.p, 2)
     got = dF.get(np.uint16, lat.shape + (2,))
     dF.free()
     solid = lat.solid().reshape(lat.shape)
-    ref = onn.bf16_bits(onn.features(solid))
+    f, ell_ref = onn.features(solid)
     lat.free()
-    assert np.array_equal(got, ref)
+    assert abs(ell - ell_ref) < 1e-12 * ell_ref
+    assert np.array_equal(got, onn.bf16_bits(f))
 
 
 def test_targets_are_the_moments_of_the_state():
@@ -353,7 +356,8 @@ def test_targets_are_the_moments_of_the_state():
     N, F = lat.N, 1e-5
     eta = 72.0 * 0.25 / 3.0
     dT = Dev(N * 16 * 2)
-    L.slb_warm_targets(lat.h, dT.p, 16)
+    ell = 2.5
+    L.slb_warm_targets(lat.h, dT.p, 16, ell)
     t = onn.from_bf16_bits(dT.get(np.uint16, (N, 16)))
     dT.free()
     L.vzero(lat.h)
@@ -369,7 +373,7 @@ def test_targets_are_the_moments_of_the_state():
     Pi = np.einsum("nl,la,lb->nab", a, cv, cv)
     s6 = np.stack([Pi[:, 0, 0] - d / 3, Pi[:, 1, 1] - d / 3, Pi[:, 2, 2] - d / 3, Pi[:, 0, 1],
                    Pi[:, 0, 2], Pi[:, 1, 2]], -1)
-    ref = np.concatenate([u * eta / F * 0.25, d[:, None] / F * 0.03125, s6 / F], -1)
+    ref = np.concatenate([u * eta / (F * ell ** 2), d[:, None] / (8 * F * ell), s6 / (F * ell)], -1)
     ref[solid] = 0.0
     assert not t[:, 10:].any()
     ok = np.abs(t[:, :10] - ref) <= np.abs(ref) * 2.0 ** -7 + 1e-6 * np.abs(ref).max(0)
@@ -423,16 +427,17 @@ def test_moment_injection_round_trip():
     assert steps < 30000
     N = A.N
     dM = Dev(N * 12 * 4)
-    L.slb_warm_moments(A.h, dM.p, 12)
+    ell = 1.7
+    L.slb_warm_moments(A.h, dM.p, 12, ell)
     m = dM.get(np.float32, (N, 12))
     assert not m[:, 10:].any()
     L.measure(A.h)
     vm = A.s.vmeanx
     # fresh lattice, same geometry: inject, compare moments, run on
     B = pack_lattice((16, 16, 24), SPHERES, F)
-    L.slb_warm_inject(B.h, dM.p, 12, 1)
+    L.slb_warm_inject(B.h, dM.p, 12, 1, ell)
     dM2 = Dev(N * 12 * 4)
-    L.slb_warm_moments(B.h, dM2.p, 12)
+    L.slb_warm_moments(B.h, dM2.p, 12, ell)
     m2 = dM2.get(np.float32, (N, 12))
     for d in (dM, dM2):
         d.free()

@@ -201,10 +201,13 @@ def test_jump_flooding_distance(shape, frac, seed):
 def test_features_signs_and_scale():
     solid = np.zeros((8, 8, 8), bool)
     solid[3:5, 3:5, 3:5] = True
-    f = onn.features(solid)
+    f, ell = onn.features(solid)
     assert np.array_equal(f[..., 0] == 1.0, solid)
     assert (f[..., 1][solid] < 0).all() and (f[..., 1][~solid] > 0).all()
+    # ell is the mean fluid distance: brute force over the fluid nodes
+    d = onn.exact_distance(solid)
+    assert abs(ell - d[~solid].mean()) < 1e-6 * ell
     # node (0, 4, 4): nearest solid is (3, 4, 4), distance 3
-    assert f[0, 4, 4, 1] == np.float32(3 * 0.125)
+    assert f[0, 4, 4, 1] == np.float32(3.0) * np.float32(1.0 / ell)
     # solid node (3, 3, 3): nearest fluid is a face neighbour
-    assert f[3, 3, 3, 1] == np.float32(-0.125)
+    assert f[3, 3, 3, 1] == -np.float32(1.0 / ell)
