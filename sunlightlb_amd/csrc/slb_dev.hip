@@ -42,7 +42,7 @@
 /* Fused step kernel.                                                      */
 
 template <typename T, bool MOVING, bool FORCE, int BF>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(512)
 k_step(const T *__restrict__ src, T *__restrict__ dst,
        const uint16_t *__restrict__ flags,
        const double *__restrict__ ux, const double *__restrict__ uy,
@@ -65,13 +65,23 @@ k_step(const T *__restrict__ src, T *__restrict__ dst,
   const unsigned fl = flags[x];
   const bool solid = (fl & FLAG_SOLID) != 0;
   T v[15];
-  v[0] = src[x];
-  Links<1, 15>::run([&](auto LL) {
-    constexpr int l = decltype(LL)::value;
-    const bool bb = (fl >> (l - 1)) & 1u;
-    const long idx = bb ? (long)OPP(l) * N + x : (long)l * N + x + nb_off<l, -1>(oi, oj, ok);
-    v[l] = src[idx];
-  });
+  if (G.nt_load) {
+    v[0] = __builtin_nontemporal_load(&src[pidx(0, x)]);
+    Links<1, 15>::run([&](auto LL) {
+      constexpr int l = decltype(LL)::value;
+      const bool bb = (fl >> (l - 1)) & 1u;
+      const long idx = bb ? pidx(OPP(l), x) : pidx(l, x + nb_off<l, -1>(oi, oj, ok));
+      v[l] = __builtin_nontemporal_load(&src[idx]);
+    });
+  } else {
+    v[0] = src[pidx(0, x)];
+    Links<1, 15>::run([&](auto LL) {
+      constexpr int l = decltype(LL)::value;
+      const bool bb = (fl >> (l - 1)) & 1u;
+      const long idx = bb ? pidx(OPP(l), x) : pidx(l, x + nb_off<l, -1>(oi, oj, ok));
+      v[l] = src[idx];
+    });
+  }
   if ((MOVING || FORCE) && (fl & FLAG_LINKS)) {
     double Fx = 0.0, Fy = 0.0, Fz = 0.0;
     Links<1, 15>::run([&](auto LL) {
@@ -87,7 +97,7 @@ k_step(const T *__restrict__ src, T *__restrict__ dst,
         if (FORCE && solid) {
           /* link m = lbar of solid x: a = a_m(x), b = a_l(x - c_l) */
           const double a = (double)v[l];
-          const double b = (double)src[(long)l * N + xs];
+          const double b = (double)src[pidx(l, xs)];
           const double g = -2.0 * (a - b + cu);
           if (CX(l)) Fx += CX(l) * g;
           if (CY(l)) Fy += CY(l) * g;
@@ -110,10 +120,10 @@ k_step(const T *__restrict__ src, T *__restrict__ dst,
   }
   if (G.nt_store) {
 #pragma unroll
-    for (int l = 0; l < 15; l++) __builtin_nontemporal_store(v[l], &dst[(long)l * N + x]);
+    for (int l = 0; l < 15; l++) __builtin_nontemporal_store(v[l], &dst[pidx(l, x)]);
   } else {
 #pragma unroll
-    for (int l = 0; l < 15; l++) dst[(long)l * N + x] = v[l];
+    for (int l = 0; l < 15; l++) dst[pidx(l, x)] = v[l];
   }
 }
 
@@ -141,14 +151,14 @@ k_bbl(T *__restrict__ a, const int32_t *__restrict__ mask,
     constexpr int l = decltype(LL)::value;
     if ((m >> (l - 1)) & 1) {
       const long f = x + nb_off<l, +1>(oi, oj, ok);
-      const double av = (double)a[(long)l * N + x];
-      const double bv = (double)a[(long)OPP(l) * N + f];
+      const double av = (double)a[pidx(l, x)];
+      const double bv = (double)a[pidx(OPP(l), f)];
       double fac = 0.0;
       if (MOVING)
         fac = (double)WL(l) * ((CX(l) ? CX(l) * ux[x] : 0.0) + (CY(l) ? CY(l) * uy[x] : 0.0) +
                                (CZ(l) ? CZ(l) * uz[x] : 0.0));
-      a[(long)l * N + x] = (T)(bv + fac);
-      a[(long)OPP(l) * N + f] = (T)(av - fac);
+      a[pidx(l, x)] = (T)(bv + fac);
+      a[pidx(OPP(l), f)] = (T)(av - fac);
       if (FORCE) {
         const double g = 2.0 * (av - bv - fac);
         if (CX(l)) Fx += CX(l) * g;
@@ -171,10 +181,10 @@ k_propagate(const T *__restrict__ src, T *__restrict__ dst, Geom G) {
   const AxisOff oi = axis_off(i, G.esi, G.plane);
   const AxisOff oj = axis_off(j, G.esj, G.esk);
   const AxisOff ok = axis_off(k, G.esk, 1);
-  dst[x] = src[x];
+  dst[pidx(0, x)] = src[pidx(0, x)];
   Links<1, 15>::run([&](auto LL) {
     constexpr int l = decltype(LL)::value;
-    dst[(long)l * N + x] = src[(long)l * N + x + nb_off<l, -1>(oi, oj, ok)];
+    dst[pidx(l, x)] = src[pidx(l, x + nb_off<l, -1>(oi, oj, ok))];
   });
 }
 
@@ -191,7 +201,7 @@ k_collide(T *__restrict__ a, const uint16_t *__restrict__ flags,
   const long N = G.N;
   T v[15];
 #pragma unroll
-  for (int l = 0; l < 15; l++) v[l] = a[(long)l * N + x];
+  for (int l = 0; l < 15; l++) v[l] = a[pidx(l, x)];
   T jbx = T(0), jby = T(0), jbz = T(0);
   if (BF != 0 && !solid) {
     jbx = T(P.bfx); jby = T(P.bfy); jbz = T(P.bfz);
@@ -199,7 +209,7 @@ k_collide(T *__restrict__ a, const uint16_t *__restrict__ flags,
   }
   collide_node<T>(v, jbx, jby, jbz, P);
 #pragma unroll
-  for (int l = 0; l < 15; l++) a[(long)l * N + x] = v[l];
+  for (int l = 0; l < 15; l++) a[pidx(l, x)] = v[l];
 }
 
 /* ---------------------------------------------------------------------- */
@@ -331,7 +341,7 @@ __device__ __forceinline__ void node_rho_j(const T *__restrict__ a, long x, long
                                            double &d, double &jx, double &jy, double &jz) {
   double v[15];
 #pragma unroll
-  for (int l = 0; l < 15; l++) v[l] = (double)a[(long)l * N + x];
+  for (int l = 0; l < 15; l++) v[l] = (double)a[pidx(l, x)];
   d = 0.0; jx = 0.0; jy = 0.0; jz = 0.0;
 #pragma unroll
   for (int l = 0; l < 15; l++) {
@@ -438,7 +448,7 @@ k_sum_pops(const T *__restrict__ a, const uint16_t *__restrict__ flags, Geom G,
        x += (long)gridDim.x * blockDim.x) {
     if (flags[x] & FLAG_SOLID) continue;
 #pragma unroll
-    for (int l = 0; l < 15; l++) s[l] += (double)a[(long)l * N + x];
+    for (int l = 0; l < 15; l++) s[l] += (double)a[pidx(l, x)];
   }
   block_reduce_store<15>(s, part);
 }
@@ -494,7 +504,7 @@ k_mprho(const T *__restrict__ a, const uint16_t *__restrict__ flags, double rho0
   double r = rho0;
   if (!(flags[x] & FLAG_SOLID)) {
 #pragma unroll
-    for (int l = 0; l < 15; l++) r += (double)a[(long)l * G.N + x];
+    for (int l = 0; l < 15; l++) r += (double)a[pidx(l, x)];
   }
   rho[x] = (T)r;
 }
@@ -560,7 +570,7 @@ k_mpstep(const T *__restrict__ a, const T *__restrict__ rho,
   /* rest link: x itself */
   if (f & 1u) {
     const T r = rho[x], P = pin[x];
-    const T pr = a[x] / r + T(16.0 / 72.0) * (r0 / r - al) + al;
+    const T pr = a[pidx(0, x)] / r + T(16.0 / 72.0) * (r0 / r - al) + al;
     np += pr * P;
     if (NMOM > 1) { n1 += pr * m1in[x]; n2 += pr * m2in[x]; }
   }
@@ -569,7 +579,7 @@ k_mpstep(const T *__restrict__ a, const T *__restrict__ rho,
     if (f & (1u << l)) {
       const long y = x + nb_off<l, -1>(oi, oj, ok);
       const T r = rho[y], P = pin[y];
-      const T pr = a[(long)l * N + y] / r + T(WL(l) / 72.0) * (r0 / r - al);
+      const T pr = a[pidx(l, y)] / r + T(WL(l) / 72.0) * (r0 / r - al);
       np += pr * P;
       if (NMOM > 1) {
         const int dd = (A.axis == 0) ? CX(l) : (A.axis == 1) ? CY(l) : CZ(l);
@@ -585,7 +595,7 @@ k_mpstep(const T *__restrict__ a, const T *__restrict__ rho,
     Links<1, 15>::run([&](auto LL) {
       constexpr int l = decltype(LL)::value;
       if (f & (1u << (16 + l))) {
-        T pr = a[(long)OPP(l) * N + x] / r + T(WL(l) / 72.0) * (r0 / r - al);
+        T pr = a[pidx(OPP(l), x)] / r + T(WL(l) / 72.0) * (r0 / r - al);
         if (MOVING) {
           const long s = x + nb_off<l, -1>(oi, oj, ok);
           const int si = (i - CX(l) + G.esi) % G.esi;
@@ -707,6 +717,25 @@ static dim3 plane_grid(const slbd_lattice *L) {
 }
 static dim3 lin_grid(long n) { return dim3((unsigned)((n + 255) / 256), 1, 1); }
 
+/* Block order of the step kernel (see step_block): plain dispatch order by
+ * default; SLB_XCD_MAP selects 1 (contiguous range per XCD) or T (T x T
+ * tiles per XCD, when T x T tiles divide the (row block, plane) grid into a
+ * multiple of 8 tiles). */
+static int step_order(const Geom &G) {
+  const long nbx = (G.plane + G.step_tpb - 1) / G.step_tpb, nby = G.esi, nb = nbx * nby;
+  auto fits = [&](int T) {
+    return T >= 2 && nbx % T == 0 && nby % T == 0 && nb % (8L * T * T) == 0;
+  };
+  const char *e = getenv("SLB_XCD_MAP");
+  if (e) {
+    const int m = atoi(e);
+    if (m == 0 || m == 1) return (m == 1 && nb % 8 == 0) ? 1 : 0;
+    return fits(m) ? m : 0;
+  }
+  /* measured on the tiled population layout: the plain order is fastest */
+  return 0;
+}
+
 extern "C" int slbd_device_count(void) {
   int n = 0;
   if (hipGetDeviceCount(&n) != hipSuccess) return 0;
@@ -731,12 +760,19 @@ extern "C" slbd_lattice *slbd_create(int esi, int esj, int esk, int prec) {
   L->esi = esi; L->esj = esj; L->esk = esk; L->prec = prec;
   L->N = (long)esi * esj * esk;
   L->G.esi = esi; L->G.esj = esj; L->G.esk = esk; L->G.plane = esj * esk; L->G.N = L->N;
-  L->G.xcd_map = getenv("SLB_XCD_MAP") ? atoi(getenv("SLB_XCD_MAP")) : 0;
   L->G.nt_store = getenv("SLB_NT_STORE") ? atoi(getenv("SLB_NT_STORE")) : 1;
+  L->G.nt_load = getenv("SLB_NT_LOAD") ? atoi(getenv("SLB_NT_LOAD")) : 0;
+  /* threads per block of the step kernel: 512 measured fastest at 256^3 */
+  L->G.step_tpb = 512;
+  if (getenv("SLB_STEP_BLOCK")) {
+    const int b = atoi(getenv("SLB_STEP_BLOCK"));
+    if (b == 256 || b == 512) L->G.step_tpb = b;
+  }
+  L->G.xcd_map = step_order(L->G);
   HIPCHECK(hipStreamCreateWithFlags(&L->stream, hipStreamNonBlocking));
   HIPCHECK(hipEventCreate(&L->ev0));
   HIPCHECK(hipEventCreate(&L->ev1));
-  const size_t pb = 15 * (size_t)L->N * tsize(prec);
+  const size_t pb = 15 * (size_t)pop_nodes(L->N) * tsize(prec);
   L->pops[0] = dalloc(pb);
   L->pops[1] = dalloc(pb);
   HIPCHECK(hipMemsetAsync(L->pops[0], 0, pb, L->stream));
@@ -783,55 +819,48 @@ extern "C" int slbd_precision(const slbd_lattice *L) { return L->prec; }
 extern "C" void slbd_sync(slbd_lattice *L) { HIPCHECK(hipStreamSynchronize(L->stream)); }
 
 extern "C" size_t slbd_bytes(const slbd_lattice *L) {
-  size_t b = 2 * 15 * (size_t)L->N * tsize(L->prec);
+  size_t b = 2 * 15 * (size_t)pop_nodes(L->N) * tsize(L->prec);
   b += (size_t)L->N * (2 + 4 + 4 + 24);
   if (L->fx) b += (size_t)L->N * 24;
   if (L->bx) b += (size_t)L->N * 3 * tsize(L->prec);
   return b;
 }
 
-/* Populations: host AoS double <-> device SoA T, transposed on the host one
- * link at a time. */
+/* Populations: host AoS double a[15n + l] <-> device tiled layout (pidx),
+ * rearranged and converted on the host, one copy each way. */
 extern "C" void slbd_put_pops(slbd_lattice *L, const double *aos) {
-  const long N = L->N;
+  const long N = L->N, Np = pop_nodes(N);
   const size_t ts = tsize(L->prec);
-  void *tmp = malloc(N * ts);
-  char *dev = (char *)L->pops[L->cur];
-  for (int l = 0; l < 15; l++) {
-    if (L->prec == 32) {
-      float *t = (float *)tmp;
-      for (long n = 0; n < N; n++) t[n] = (float)aos[15 * n + l];
-    } else {
-      double *t = (double *)tmp;
-      for (long n = 0; n < N; n++) t[n] = aos[15 * n + l];
+  void *tmp = calloc((size_t)15 * Np, ts);
+  if (!tmp) { fprintf(stderr, "sunlightlb: out of host memory\n"); exit(EXIT_FAILURE); }
+  for (long n = 0; n < N; n++)
+    for (int l = 0; l < 15; l++) {
+      if (L->prec == 32) ((float *)tmp)[pidx(l, n)] = (float)aos[15 * n + l];
+      else ((double *)tmp)[pidx(l, n)] = aos[15 * n + l];
     }
-    HIPCHECK(hipMemcpyAsync(dev + (size_t)l * N * ts, tmp, N * ts, hipMemcpyHostToDevice, L->stream));
-    HIPCHECK(hipStreamSynchronize(L->stream));
-  }
+  HIPCHECK(hipMemcpyAsync(L->pops[L->cur], tmp, (size_t)15 * Np * ts, hipMemcpyHostToDevice,
+                          L->stream));
+  HIPCHECK(hipStreamSynchronize(L->stream));
   free(tmp);
 }
 
 extern "C" void slbd_get_pops(slbd_lattice *L, double *aos) {
-  const long N = L->N;
+  const long N = L->N, Np = pop_nodes(N);
   const size_t ts = tsize(L->prec);
-  void *tmp = malloc(N * ts);
-  const char *dev = (const char *)L->pops[L->cur];
-  for (int l = 0; l < 15; l++) {
-    HIPCHECK(hipMemcpyAsync(tmp, dev + (size_t)l * N * ts, N * ts, hipMemcpyDeviceToHost, L->stream));
-    HIPCHECK(hipStreamSynchronize(L->stream));
-    if (L->prec == 32) {
-      const float *t = (const float *)tmp;
-      for (long n = 0; n < N; n++) aos[15 * n + l] = (double)t[n];
-    } else {
-      const double *t = (const double *)tmp;
-      for (long n = 0; n < N; n++) aos[15 * n + l] = t[n];
-    }
-  }
+  void *tmp = malloc((size_t)15 * Np * ts);
+  if (!tmp) { fprintf(stderr, "sunlightlb: out of host memory\n"); exit(EXIT_FAILURE); }
+  HIPCHECK(hipMemcpyAsync(tmp, L->pops[L->cur], (size_t)15 * Np * ts, hipMemcpyDeviceToHost,
+                          L->stream));
+  HIPCHECK(hipStreamSynchronize(L->stream));
+  for (long n = 0; n < N; n++)
+    for (int l = 0; l < 15; l++)
+      aos[15 * n + l] = L->prec == 32 ? (double)((const float *)tmp)[pidx(l, n)]
+                                      : ((const double *)tmp)[pidx(l, n)];
   free(tmp);
 }
 
 extern "C" void slbd_zero_pops(slbd_lattice *L) {
-  HIPCHECK(hipMemsetAsync(L->pops[L->cur], 0, 15 * (size_t)L->N * tsize(L->prec), L->stream));
+  HIPCHECK(hipMemsetAsync(L->pops[L->cur], 0, 15 * (size_t)pop_nodes(L->N) * tsize(L->prec), L->stream));
   HIPCHECK(hipStreamSynchronize(L->stream));
 }
 
@@ -846,7 +875,7 @@ extern "C" int slbd_set_precision(slbd_lattice *L, int prec, double *scratch_aos
   L->bx = L->by = L->bz = NULL;
   free_mp(L);
   L->prec = prec;
-  const size_t pb = 15 * (size_t)L->N * tsize(prec);
+  const size_t pb = 15 * (size_t)pop_nodes(L->N) * tsize(prec);
   L->pops[0] = dalloc(pb);
   L->pops[1] = dalloc(pb);
   L->cur = 0;
@@ -964,7 +993,7 @@ extern "C" void slbd_scale_f(slbd_lattice *L, double s) {
   }
 }
 extern "C" void slbd_scale_pops(slbd_lattice *L, double s) {
-  const long n = 15 * L->N;
+  const long n = 15 * pop_nodes(L->N);
   if (L->prec == 32)
     hipLaunchKernelGGL(k_scale<float>, lin_grid(n), dim3(256), 0, L->stream,
                        (float *)L->pops[L->cur], n, s);
@@ -995,7 +1024,8 @@ template <typename T, bool MOVING, bool FORCE, int BF>
 static void launch_step_t(slbd_lattice *L, const PhysArgs &A) {
   const T *src = (const T *)L->pops[L->cur];
   T *dst = (T *)L->pops[L->cur ^ 1];
-  hipLaunchKernelGGL((k_step<T, MOVING, FORCE, BF>), plane_grid(L), dim3(256), 0, L->stream,
+  const dim3 grid((unsigned)((L->G.plane + L->G.step_tpb - 1) / L->G.step_tpb), (unsigned)L->esi);
+  hipLaunchKernelGGL((k_step<T, MOVING, FORCE, BF>), grid, dim3(L->G.step_tpb), 0, L->stream,
                      src, dst, L->flags, L->ux, L->uy, L->uz, (const T *)L->bx,
                      (const T *)L->by, (const T *)L->bz, L->fx, L->fy, L->fz, L->G, A);
   LAUNCHCHECK();
@@ -1553,7 +1583,7 @@ __global__ void k_targets(const T *__restrict__ a, const uint16_t *__restrict__ 
     double d = 0, j[3] = {0, 0, 0}, P[6] = {0, 0, 0, 0, 0, 0};
 #pragma unroll
     for (int l = 0; l < 15; l++) {
-      const double v = (double)a[(long)l * G.N + n];
+      const double v = (double)a[pidx(l, n)];
       d += v;
       j[0] += CX(l) * v; j[1] += CY(l) * v; j[2] += CZ(l) * v;
       P[0] += CX(l) * CX(l) * v; P[1] += CY(l) * CY(l) * v; P[2] += CZ(l) * CZ(l) * v;
@@ -1630,7 +1660,7 @@ __global__ void k_inject(T *__restrict__ a, const uint16_t *__restrict__ flags,
     for (int p = 0; p < 3; p++)
       for (int q = 0; q < 3; q++) cs += c[p] * c[q] * s[p][q];
     const double cj = c[0] * j[0] + c[1] * j[1] + c[2] * j[2];
-    a[(long)l * G.N + n] = (T)((WL(l) / 72.0) * (d + 3.0 * cj + 4.5 * (cs - tr / 3.0)));
+    a[pidx(l, n)] = (T)((WL(l) / 72.0) * (d + 3.0 * cj + 4.5 * (cs - tr / 3.0)));
   }
 }
 

@@ -46,6 +46,25 @@ __host__ __device__ constexpr int OPP(int l) {
   return l == 0 ? 0 : (l <= 6 ? ((l & 1) ? l + 1 : l - 1) : 21 - l);
 }
 
+/* Population layout in HBM: tiles of POP_TILE consecutive nodes, each tile
+ * holding the 15 populations as 15 contiguous runs of POP_TILE values
+ * (structure of arrays within a tile).  A block's own populations and
+ * those of its row neighbours then lie in a few contiguous 60-120 KB
+ * stretches rather than in 15 arrays N apart, so the bounce-back loads of
+ * a node's own opposite populations stay within the pages and lines the
+ * pull is already touching.  Allocations hold pop_nodes(N) nodes (N rounded
+ * up to a whole tile). */
+#ifndef POP_SHIFT
+#define POP_SHIFT 10
+#endif
+#define POP_TILE (1L << POP_SHIFT)
+__host__ __device__ __forceinline__ long pidx(int l, long x) {
+  return ((x >> POP_SHIFT) * 15L + l) * POP_TILE + (x & (POP_TILE - 1));
+}
+__host__ __device__ __forceinline__ long pop_nodes(long N) {
+  return (N + POP_TILE - 1) & ~(POP_TILE - 1);
+}
+
 #define FLAG_SOLID 0x4000u
 #define FLAG_LINKS 0x3fffu
 
@@ -76,18 +95,29 @@ __device__ __forceinline__ long nb_off(const AxisOff &oi, const AxisOff &oj, con
 struct Geom {
   int esi, esj, esk, plane;
   long N;
-  int xcd_map;   /* step kernel: give each XCD a contiguous range of rows */
+  int xcd_map;   /* step kernel block order: 0 plain, 1 contiguous range per
+                    XCD, T >= 2 tiles of T x T rows per XCD */
   int nt_store;  /* step kernel: non-temporal population stores */
+  int nt_load;   /* step kernel: non-temporal population loads */
+  int step_tpb;  /* step kernel: threads per block (256 or 512) */
 };
 
-/* Block coordinates of the step kernel.  Workgroups are dispatched round
- * robin over the 8 XCDs; with xcd_map the linear block id b is served by
- * XCD b % 8 and that XCD walks a contiguous 1/8 of the (row, plane) space,
- * so blocks that share boundary rows share an L2. */
+/* Block coordinates of the step kernel (blockIdx.x: block of rows within a
+ * plane, blockIdx.y: plane i).  Workgroups are dispatched round robin over
+ * the 8 XCDs, each with its own L2.  A bounce-back link makes a node read
+ * its own opposite population, a cache line that the row on the other side
+ * of the link also reads; when the two rows run on different XCDs, or far
+ * apart in time, the line is fetched from HBM twice.
+ *   xcd_map == 1: XCD b % 8 walks a contiguous 1/8 of the rows.
+ *   xcd_map == T: the rows are cut in T x T tiles of (plane, row block);
+ *   XCD x takes tiles x, x + 8, x + 16, ... in dispatch order and walks each
+ *   tile row block by row block, so both neighbours of a row are within T
+ *   blocks on the same XCD, while the 8 XCDs stay in neighbouring tiles.
+ * The caller sets xcd_map only when the grid divides (slbd_step_order). */
 __device__ __forceinline__ void step_block(const Geom &G, int &bx, int &by) {
   bx = blockIdx.x;
   by = blockIdx.y;
-  if (G.xcd_map) {
+  if (G.xcd_map == 1) {
     const int nbx = gridDim.x, nb = gridDim.x * gridDim.y;
     if ((nb & 7) == 0) {
       const int lin = by * nbx + bx;
@@ -95,6 +125,15 @@ __device__ __forceinline__ void step_block(const Geom &G, int &bx, int &by) {
       bx = lin2 % nbx;
       by = lin2 / nbx;
     }
+  } else if (G.xcd_map >= 2) {
+    const int T = G.xcd_map, TT = T * T, nbx = gridDim.x;
+    const int lin = by * nbx + bx;
+    const int k = lin >> 3;
+    const int tile = (k / TT) * 8 + (lin & 7), w = k - (k / TT) * TT;
+    const int ntx = nbx / T;
+    const int ti = tile / ntx, tj = tile - ti * ntx;
+    by = ti * T + w / T;
+    bx = tj * T + (w - (w / T) * T);
   }
 }
 
