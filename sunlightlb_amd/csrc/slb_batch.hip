@@ -26,7 +26,8 @@
 struct BProb {
   int esi, esj, esk, plane;
   long N;
-  long pop_off;   /* first population of this problem in the arena */
+  long pop_off;   /* first population of this problem in the arena (tiled
+                     layout pidx, whole tiles per problem) */
   long node_off;  /* first node of this problem's box in the node tables */
   int tab_off;    /* first object row (9 doubles each) */
   int nobj;
@@ -65,7 +66,6 @@ __global__ void k_bstep(const T *__restrict__ src, T *__restrict__ dst,
     for (int q = 0; q < NPT; q++) {
       const long x = (long)td.y + (long)q * blockDim.x + threadIdx.x;
       if (x >= P.N) continue;
-      const long N = P.N;
       /* N < 2^31 per problem: 32-bit division */
       const int xi = (int)x;
       const int i = xi / P.plane;
@@ -77,11 +77,11 @@ __global__ void k_bstep(const T *__restrict__ src, T *__restrict__ dst,
       const unsigned f = fl[x];
       const bool solid = (f & FLAG_SOLID) != 0;
       T v[15];
-      v[0] = s[x];
+      v[0] = s[pidx(0, x)];
       Links<1, 15>::run([&](auto LL) {
         constexpr int l = decltype(LL)::value;
         const bool bb = (f >> (l - 1)) & 1u;
-        const long idx = bb ? (long)OPP(l) * N + x : (long)l * N + x + nb_off<l, -1>(oi, oj, ok);
+        const long idx = bb ? pidx(OPP(l), x) : pidx(l, x + nb_off<l, -1>(oi, oj, ok));
         v[l] = s[idx];
       });
       if ((P.moving || FORCE) && (f & FLAG_LINKS)) {
@@ -108,7 +108,7 @@ __global__ void k_bstep(const T *__restrict__ src, T *__restrict__ dst,
             }
             if (FORCE && solid) {
               const double a = (double)v[l];
-              const double b = (double)s[(long)l * N + xs];
+              const double b = (double)s[pidx(l, xs)];
               const double g = -2.0 * (a - b + cu);
               if (CX(l)) Fx += CX(l) * g;
               if (CY(l)) Fy += CY(l) * g;
@@ -133,7 +133,7 @@ __global__ void k_bstep(const T *__restrict__ src, T *__restrict__ dst,
         collide_node<T>(v, jbx, jby, jbz, A);
       }
 #pragma unroll
-      for (int l = 0; l < 15; l++) d[(long)l * N + x] = v[l];
+      for (int l = 0; l < 15; l++) d[pidx(l, x)] = v[l];
     }
     if (lds) {
       __syncthreads();
@@ -274,7 +274,7 @@ extern "C" int slbd_batch_commit(slbd_batch *B) {
     const long N = (long)B->box[b].esi * B->box[b].esj * B->box[b].esk;
     nprob += B->box[b].nrhs;
     nodes += N;
-    pops += 15 * N * B->box[b].nrhs;
+    pops += 15 * pop_nodes(N) * B->box[b].nrhs;
     ntab += (long)B->box[b].nobj * B->box[b].nrhs;
     nacc += B->box[b].nobj * B->box[b].nrhs;
   }
@@ -323,7 +323,7 @@ extern "C" int slbd_batch_commit(slbd_batch *B) {
       q->moving = x->moving; q->stokes = x->stokes; q->skip_solid = x->skip_solid;
       q->omega = 1.0 / x->tau; q->comega = 1.0 - q->omega; q->rho0 = x->rho0;
       for (int c = 0; c < 3; c++) q->bf[c] = x->bf[c];
-      pop_off += 15 * N;
+      pop_off += 15 * pop_nodes(N);
     }
     node_off += N;
     tab_off += x->nobj * x->nrhs;
@@ -447,13 +447,14 @@ extern "C" int slbd_batch_get_pops(slbd_batch *B, int box, int rhs, double *aos)
   HIPCHECK(hipStreamSynchronize(B->stream));
   const BProb *q = &B->hprobs[B->box_first_prob[box] + rhs];
   const size_t ts = bts(B->prec);
-  char *tmp = (char *)malloc(15 * q->N * ts);
-  HIPCHECK(hipMemcpy(tmp, (char *)B->pops[B->cur] + q->pop_off * ts, 15 * q->N * ts,
+  const long np = 15 * pop_nodes(q->N);
+  char *tmp = (char *)malloc(np * ts);
+  HIPCHECK(hipMemcpy(tmp, (char *)B->pops[B->cur] + q->pop_off * ts, np * ts,
                      hipMemcpyDeviceToHost));
   for (int l = 0; l < 15; l++)
     for (long n = 0; n < q->N; n++)
-      aos[15 * n + l] = (B->prec == 32) ? (double)((float *)tmp)[l * q->N + n]
-                                        : ((double *)tmp)[l * q->N + n];
+      aos[15 * n + l] = (B->prec == 32) ? (double)((float *)tmp)[pidx(l, n)]
+                                        : ((double *)tmp)[pidx(l, n)];
   free(tmp);
   return 0;
 }

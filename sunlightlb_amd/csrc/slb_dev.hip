@@ -541,7 +541,12 @@ struct MPArgs {
   int axis;
 };
 
-/* One moment-propagation step for every node x:
+/* MP fields live in node records of W = 2 (P, 1/rho) or W = 4 (P, m1, m2,
+ * 1/rho) values, so that the gather below reads one vector per incoming link
+ * instead of four scalars; 1/rho (the lattice's frozen density) is kept in
+ * each record and refreshed whenever the density changes.
+ *
+ * One moment-propagation step for every node x:
  *   new(x) = sum_l [x - c_l fluid] pi_l(x - c_l) P(x - c_l)
  *          + sum_l [x - c_l reflecting solid] (1 - pads)(pi_lbar(x)
  *                 + w_l c_l.U(x - c_l)/rho(x)) P(x)
@@ -550,52 +555,54 @@ struct MPArgs {
  * site accumulate, other solid clear; partial adsorption keeps pads*new). */
 template <typename T, int NMOM, bool MOVING>
 __global__ void __launch_bounds__(256)
-k_mpstep(const T *__restrict__ a, const T *__restrict__ rho,
-         const T *__restrict__ pin, T *__restrict__ pout,
-         const T *__restrict__ m1in, T *__restrict__ m1out,
-         const T *__restrict__ m2in, T *__restrict__ m2out,
+k_mpstep(const T *__restrict__ a, const T *__restrict__ qin, T *__restrict__ qout,
          const uint32_t *__restrict__ mpf, const int32_t *__restrict__ mask,
          const int32_t *__restrict__ objidx, const double *__restrict__ tab,
          Geom G, MPArgs A) {
+  constexpr int W = NMOM > 1 ? 4 : 2;
+  typedef T V __attribute__((ext_vector_type(W)));
+  const V *__restrict__ Q = (const V *)qin;
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= G.plane) return;
   const int i = blockIdx.y, j = t / G.esk, k = t - j * G.esk;
-  const long x = (long)i * G.plane + t, N = G.N;
+  const long x = (long)i * G.plane + t;
   const AxisOff oi = axis_off(i, G.esi, G.plane);
   const AxisOff oj = axis_off(j, G.esj, G.esk);
   const AxisOff ok = axis_off(k, G.esk, 1);
   const uint32_t f = mpf[x];
   const T r0 = T(A.rho0), al = T(A.alpha);
+  const V qx = Q[x];
+  const T rix = qx[W - 1];
   T np = T(0), n1 = T(0), n2 = T(0);
   /* rest link: x itself */
   if (f & 1u) {
-    const T r = rho[x], P = pin[x];
-    const T pr = a[pidx(0, x)] / r + T(16.0 / 72.0) * (r0 / r - al) + al;
-    np += pr * P;
-    if (NMOM > 1) { n1 += pr * m1in[x]; n2 += pr * m2in[x]; }
+    const T pr = a[pidx(0, x)] * rix + T(16.0 / 72.0) * (r0 * rix - al) + al;
+    np += pr * qx[0];
+    if (NMOM > 1) { n1 += pr * qx[1]; n2 += pr * qx[2]; }
   }
   Links<1, 15>::run([&](auto LL) {
     constexpr int l = decltype(LL)::value;
     if (f & (1u << l)) {
       const long y = x + nb_off<l, -1>(oi, oj, ok);
-      const T r = rho[y], P = pin[y];
-      const T pr = a[pidx(l, y)] / r + T(WL(l) / 72.0) * (r0 / r - al);
+      const V qy = Q[y];
+      const T ri = qy[W - 1], P = qy[0];
+      const T pr = a[pidx(l, y)] * ri + T(WL(l) / 72.0) * (r0 * ri - al);
       np += pr * P;
       if (NMOM > 1) {
         const int dd = (A.axis == 0) ? CX(l) : (A.axis == 1) ? CY(l) : CZ(l);
-        const T q1 = m1in[y];
+        const T q1 = qy[1];
         n1 += pr * (q1 + T(dd) * P);
-        n2 += pr * (m2in[y] + T(2 * dd) * q1 + T(dd * dd) * P);
+        n2 += pr * (qy[2] + T(2 * dd) * q1 + T(dd * dd) * P);
       }
     }
   });
   if (f >> 17) {
-    const T r = rho[x], P = pin[x];
+    const T P = qx[0];
     const T keep = T(1.0 - A.pads);
     Links<1, 15>::run([&](auto LL) {
       constexpr int l = decltype(LL)::value;
       if (f & (1u << (16 + l))) {
-        T pr = a[pidx(OPP(l), x)] / r + T(WL(l) / 72.0) * (r0 / r - al);
+        T pr = a[pidx(OPP(l), x)] * rix + T(WL(l) / 72.0) * (r0 * rix - al);
         if (MOVING) {
           const long s = x + nb_off<l, -1>(oi, oj, ok);
           const int si = (i - CX(l) + G.esi) % G.esi;
@@ -604,50 +611,76 @@ k_mpstep(const T *__restrict__ a, const T *__restrict__ rho,
           double Ux, Uy, Uz;
           obj_velocity(tab, objidx[s], si, sj, sk, A.rho0, Ux, Uy, Uz);
           const double cu = (double)WL(l) * (CX(l) * Ux + CY(l) * Uy + CZ(l) * Uz);
-          pr += (T)(cu / (double)r);
+          pr += (T)(cu * (double)rix);
         }
         pr *= keep;
         np += pr * P;
-        if (NMOM > 1) { n1 += pr * m1in[x]; n2 += pr * m2in[x]; }
+        if (NMOM > 1) { n1 += pr * qx[1]; n2 += pr * qx[2]; }
       }
     });
   }
-  T o0, o1 = T(0), o2 = T(0);
+  V o;
+  o[W - 1] = rix;
+  o[0] = T(0);
+  if (NMOM > 1) { o[1] = T(0); o[2] = T(0); }
   if (f & 1u) {
-    o0 = np;
-    if (NMOM > 1) { o1 = n1; o2 = n2; }
+    o[0] = np;
+    if (NMOM > 1) { o[1] = n1; o[2] = n2; }
   } else {
     const int m = mask[x];
     T frac = T(0);
     if (m & 0x4000) frac = T(1);
     else if (m != 0 && A.pads > 0.0) frac = T(A.pads);
     if (frac != T(0)) {
-      o0 = pin[x] + frac * np;
-      if (NMOM > 1) { o1 = m1in[x] + frac * n1; o2 = m2in[x] + frac * n2; }
-    } else {
-      o0 = T(0);
+      o[0] = qx[0] + frac * np;
+      if (NMOM > 1) { o[1] = qx[1] + frac * n1; o[2] = qx[2] + frac * n2; }
     }
   }
-  pout[x] = o0;
-  if (NMOM > 1) { m1out[x] = o1; m2out[x] = o2; }
+  ((V *)qout)[x] = o;
+}
+
+/* record component access: q[x * W + c] */
+template <typename T>
+__global__ void k_rec_put(T *__restrict__ q, int W, int c, const double *__restrict__ v, long N) {
+  const long x = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (x < N) q[x * W + c] = (T)v[x];
+}
+template <typename T>
+__global__ void k_rec_get(const T *__restrict__ q, int W, int c, double *__restrict__ v, long N) {
+  const long x = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (x < N) v[x] = (double)q[x * W + c];
+}
+template <typename T>
+__global__ void k_rec_zero(T *__restrict__ q, int W, int c, long N) {
+  const long x = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (x < N) q[x * W + c] = T(0);
+}
+/* 1/rho into the last slot of both record buffers */
+template <typename T>
+__global__ void k_rec_rinv(T *__restrict__ q0, T *__restrict__ q1, int W,
+                           const T *__restrict__ rho, long N) {
+  const long x = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= N) return;
+  const T r = T(1) / rho[x];
+  q0[x * W + W - 1] = r;
+  q1[x * W + W - 1] = r;
 }
 
 template <typename T>
 __global__ void __launch_bounds__(256)
-k_mpsums(const T *__restrict__ p, const T *__restrict__ m1, const T *__restrict__ m2,
-         const uint16_t *__restrict__ flags, const int32_t *__restrict__ mask,
-         int nmom, Geom G, double *__restrict__ part) {
+k_mpsums(const T *__restrict__ q, int W, const uint16_t *__restrict__ flags,
+         const int32_t *__restrict__ mask, int nmom, Geom G, double *__restrict__ part) {
   double s[6] = {0, 0, 0, 0, 0, 0};
   const long N = G.N;
   for (long x = (long)blockIdx.x * blockDim.x + threadIdx.x; x < N;
        x += (long)gridDim.x * blockDim.x) {
-    const double v = (double)p[x];
+    const double v = (double)q[x * W];
     if (flags[x] & FLAG_SOLID) {
       s[1] += v;
     } else {
       s[0] += v;
       s[5] += 1.0;
-      if (nmom > 1) { s[3] += (double)m1[x]; s[4] += (double)m2[x]; }
+      if (nmom > 1) { s[3] += (double)q[x * W + 1]; s[4] += (double)q[x * W + 2]; }
     }
     if (mask[x] & 0x4000) s[2] += v;
   }
@@ -673,6 +706,7 @@ __global__ void k_scale(T *__restrict__ d, long n, double s) {
 
 struct slbd_lattice {
   int esi, esj, esk, prec;
+  long rho_version;   /* bumped whenever rho changes (MP 1/rho slots) */
   long N;
   Geom G;
   hipStream_t stream;
@@ -698,9 +732,10 @@ struct slbd_lattice {
 };
 
 struct slbd_mp {
-  void *f[3][2];   /* P, m1, m2 ping-pong pairs */
-  int cur, nmom, prec;
+  void *q[2];      /* node records {P, (m1, m2,) 1/rho}, ping-pong pair */
+  int cur, nmom, prec, W;
   long N;
+  long rho_version;  /* lattice density version the 1/rho slots hold */
 };
 
 static size_t tsize(int prec) { return prec == 32 ? sizeof(float) : sizeof(double); }
@@ -798,6 +833,7 @@ extern "C" slbd_lattice *slbd_create(int esi, int esj, int esk, int prec) {
 static void free_mp(slbd_lattice *L) {
   if (L->rho) { HIPCHECK(hipFree(L->rho)); L->rho = NULL; }
   if (L->mpf) { HIPCHECK(hipFree(L->mpf)); L->mpf = NULL; }
+  L->rho_version++;
 }
 
 extern "C" void slbd_destroy(slbd_lattice *L) {
@@ -1293,14 +1329,15 @@ extern "C" slbd_mp *slbd_mp_create(slbd_lattice *L, int nmom) {
   ensure_mp_lattice(L);
   slbd_mp *M = (slbd_mp *)calloc(1, sizeof(slbd_mp));
   M->nmom = nmom > 1 ? 3 : 1;
+  M->W = M->nmom > 1 ? 4 : 2;
   M->prec = L->prec;
   M->N = L->N;
-  const size_t ts = tsize(L->prec);
-  for (int f = 0; f < M->nmom; f++)
-    for (int b = 0; b < 2; b++) {
-      M->f[f][b] = dalloc(L->N * ts);
-      HIPCHECK(hipMemsetAsync(M->f[f][b], 0, L->N * ts, L->stream));
-    }
+  M->rho_version = -1;
+  const size_t bytes = (size_t)L->N * M->W * tsize(L->prec);
+  for (int b = 0; b < 2; b++) {
+    M->q[b] = dalloc(bytes);
+    HIPCHECK(hipMemsetAsync(M->q[b], 0, bytes, L->stream));
+  }
   HIPCHECK(hipStreamSynchronize(L->stream));
   return M;
 }
@@ -1308,9 +1345,8 @@ extern "C" slbd_mp *slbd_mp_create(slbd_lattice *L, int nmom) {
 extern "C" void slbd_mp_destroy(slbd_mp *M) {
   if (!M) return;
   HIPCHECK(hipDeviceSynchronize());
-  for (int f = 0; f < 3; f++)
-    for (int b = 0; b < 2; b++)
-      if (M->f[f][b]) HIPCHECK(hipFree(M->f[f][b]));
+  for (int b = 0; b < 2; b++)
+    if (M->q[b]) HIPCHECK(hipFree(M->q[b]));
   free(M);
 }
 
@@ -1322,6 +1358,19 @@ static void check_mp(const slbd_lattice *L, const slbd_mp *M) {
   }
 }
 
+/* refresh the 1/rho slots when the lattice density changed */
+static void mp_sync_rho(slbd_lattice *L, slbd_mp *M) {
+  if (M->rho_version == L->rho_version) return;
+  if (L->prec == 32)
+    hipLaunchKernelGGL(k_rec_rinv<float>, lin_grid(L->N), dim3(256), 0, L->stream,
+                       (float *)M->q[0], (float *)M->q[1], M->W, (const float *)L->rho, L->N);
+  else
+    hipLaunchKernelGGL(k_rec_rinv<double>, lin_grid(L->N), dim3(256), 0, L->stream,
+                       (double *)M->q[0], (double *)M->q[1], M->W, (const double *)L->rho, L->N);
+  LAUNCHCHECK();
+  M->rho_version = L->rho_version;
+}
+
 extern "C" void slbd_mp_rhoinit(slbd_lattice *L, double rho0) {
   ensure_mp_lattice(L);
   if (L->prec == 32)
@@ -1331,6 +1380,7 @@ extern "C" void slbd_mp_rhoinit(slbd_lattice *L, double rho0) {
     hipLaunchKernelGGL(k_mprho<double>, lin_grid(L->N), dim3(256), 0, L->stream,
                        (const double *)L->pops[L->cur], L->flags, rho0, (double *)L->rho, L->G);
   LAUNCHCHECK();
+  L->rho_version++;
 }
 
 static void put_field_T(slbd_lattice *L, void *dev, const double *host) {
@@ -1351,6 +1401,7 @@ static void get_field_T(slbd_lattice *L, double *host, const void *dev) {
 extern "C" void slbd_mp_put_rho(slbd_lattice *L, const double *rho) {
   ensure_mp_lattice(L);
   put_field_T(L, L->rho, rho);
+  L->rho_version++;
 }
 extern "C" void slbd_mp_get_rho(slbd_lattice *L, double *rho) {
   ensure_mp_lattice(L);
@@ -1359,27 +1410,50 @@ extern "C" void slbd_mp_get_rho(slbd_lattice *L, double *rho) {
 extern "C" void slbd_mp_put(slbd_lattice *L, slbd_mp *M, int field, const double *v) {
   check_mp(L, M);
   if (field >= M->nmom) return;
-  put_field_T(L, M->f[field][M->cur], v);
+  double *stage = (double *)dalloc(sizeof(double) * L->N);
+  copy_in(stage, v, sizeof(double) * L->N, L->stream);
+  if (L->prec == 32)
+    hipLaunchKernelGGL(k_rec_put<float>, lin_grid(L->N), dim3(256), 0, L->stream,
+                       (float *)M->q[M->cur], M->W, field, stage, L->N);
+  else
+    hipLaunchKernelGGL(k_rec_put<double>, lin_grid(L->N), dim3(256), 0, L->stream,
+                       (double *)M->q[M->cur], M->W, field, stage, L->N);
+  LAUNCHCHECK();
+  HIPCHECK(hipStreamSynchronize(L->stream));
+  HIPCHECK(hipFree(stage));
 }
 extern "C" void slbd_mp_get(slbd_lattice *L, slbd_mp *M, int field, double *v) {
   check_mp(L, M);
   if (field >= M->nmom) { memset(v, 0, sizeof(double) * L->N); return; }
-  get_field_T(L, v, M->f[field][M->cur]);
+  double *stage = (double *)dalloc(sizeof(double) * L->N);
+  if (L->prec == 32)
+    hipLaunchKernelGGL(k_rec_get<float>, lin_grid(L->N), dim3(256), 0, L->stream,
+                       (const float *)M->q[M->cur], M->W, field, stage, L->N);
+  else
+    hipLaunchKernelGGL(k_rec_get<double>, lin_grid(L->N), dim3(256), 0, L->stream,
+                       (const double *)M->q[M->cur], M->W, field, stage, L->N);
+  LAUNCHCHECK();
+  copy_out(v, stage, sizeof(double) * L->N, L->stream);
+  HIPCHECK(hipFree(stage));
 }
 extern "C" void slbd_mp_zero(slbd_lattice *L, slbd_mp *M, int field) {
   check_mp(L, M);
   if (field >= M->nmom) return;
-  HIPCHECK(hipMemsetAsync(M->f[field][M->cur], 0, L->N * tsize(L->prec), L->stream));
+  if (L->prec == 32)
+    hipLaunchKernelGGL(k_rec_zero<float>, lin_grid(L->N), dim3(256), 0, L->stream,
+                       (float *)M->q[M->cur], M->W, field, L->N);
+  else
+    hipLaunchKernelGGL(k_rec_zero<double>, lin_grid(L->N), dim3(256), 0, L->stream,
+                       (double *)M->q[M->cur], M->W, field, L->N);
+  LAUNCHCHECK();
 }
 
 template <typename T, int NMOM, bool MOVING>
 static void launch_mp_t(slbd_lattice *L, slbd_mp *M, const MPArgs &A) {
   const int c = M->cur;
   hipLaunchKernelGGL((k_mpstep<T, NMOM, MOVING>), plane_grid(L), dim3(256), 0, L->stream,
-                     (const T *)L->pops[L->cur], (const T *)L->rho, (const T *)M->f[0][c],
-                     (T *)M->f[0][c ^ 1], (const T *)M->f[1][c], (T *)M->f[1][c ^ 1],
-                     (const T *)M->f[2][c], (T *)M->f[2][c ^ 1], L->mpf, L->mask, L->objidx,
-                     L->tab, L->G, A);
+                     (const T *)L->pops[L->cur], (const T *)M->q[c], (T *)M->q[c ^ 1], L->mpf,
+                     L->mask, L->objidx, L->tab, L->G, A);
   LAUNCHCHECK();
   M->cur ^= 1;
 }
@@ -1399,6 +1473,7 @@ extern "C" void slbd_mp_step(slbd_lattice *L, slbd_mp *M, double rho0, double al
   (void)nmom;
   check_mp(L, M);
   ensure_mp_lattice(L);
+  mp_sync_rho(L, M);
   if (moving) put_tab(L, nobj, objtab);
   MPArgs A;
   A.rho0 = rho0; A.alpha = alpha; A.pads = pads; A.axis = axis;
@@ -1410,6 +1485,9 @@ extern "C" void slbd_mp_step(slbd_lattice *L, slbd_mp *M, double rho0, double al
 
 extern "C" double slbd_time_mp(slbd_lattice *L, slbd_mp *M, double rho0, double alpha,
                                double pads, int nmom, int nsteps) {
+  check_mp(L, M);
+  ensure_mp_lattice(L);
+  mp_sync_rho(L, M);
   HIPCHECK(hipStreamSynchronize(L->stream));
   HIPCHECK(hipEventRecord(L->ev0, L->stream));
   slbd_mp_step(L, M, rho0, alpha, 0, 0, NULL, pads, nmom, 0, nsteps);
@@ -1426,12 +1504,10 @@ extern "C" void slbd_mp_sums(slbd_lattice *L, slbd_mp *M, double out[6]) {
   const int c = M->cur;
   if (L->prec == 32)
     hipLaunchKernelGGL(k_mpsums<float>, dim3(nb), dim3(256), 0, L->stream,
-                       (const float *)M->f[0][c], (const float *)M->f[1][c],
-                       (const float *)M->f[2][c], L->flags, L->mask, M->nmom, L->G, L->part);
+                       (const float *)M->q[c], M->W, L->flags, L->mask, M->nmom, L->G, L->part);
   else
     hipLaunchKernelGGL(k_mpsums<double>, dim3(nb), dim3(256), 0, L->stream,
-                       (const double *)M->f[0][c], (const double *)M->f[1][c],
-                       (const double *)M->f[2][c], L->flags, L->mask, M->nmom, L->G, L->part);
+                       (const double *)M->q[c], M->W, L->flags, L->mask, M->nmom, L->G, L->part);
   LAUNCHCHECK();
   hipLaunchKernelGGL(k_final<6>, dim3(1), dim3(256), 0, L->stream, L->part, nb, L->dout);
   LAUNCHCHECK();
