@@ -34,13 +34,15 @@ struct BProb {
   int acc_off;    /* first object accumulator (6 doubles each) */
   int moving, stokes, skip_solid;
   double omega, comega, rho0;
+  double uscale;  /* rho0 / 12: wall velocity scale of vspread */
   double bf[3];
 };
 
 #define BATCH_LDS_OBJ 64
 
 template <typename T, bool FORCE, int NPT>
-__global__ void k_bstep(const T *__restrict__ src, T *__restrict__ dst,
+__global__ void __launch_bounds__(512)
+k_bstep(const T *__restrict__ src, T *__restrict__ dst,
                         const uint16_t *__restrict__ flags, const int32_t *__restrict__ objidx,
                         const double *__restrict__ tab, const BProb *__restrict__ probs,
                         const int2 *__restrict__ tiles, int ntiles, double *__restrict__ acc) {
@@ -92,7 +94,10 @@ __global__ void k_bstep(const T *__restrict__ src, T *__restrict__ dst,
             const long xs = x + nb_off<l, -1>(oi, oj, ok);
             double cu = 0.0;
             if (P.moving) {
-              /* wall velocity of the solid end of the link */
+              /* wall velocity U = (rho0/12)(v + w x r) of the solid end of
+               * the link (vspread, src/lbobjs.c:565-589, r without minimum
+               * image); only its component along c_l enters:
+               * c.U = (rho0/12)(c.v + c.(w x r)) */
               long sn = x;
               int si = i, sj = j, sk = k;
               if (!solid) {
@@ -101,10 +106,13 @@ __global__ void k_bstep(const T *__restrict__ src, T *__restrict__ dst,
                 sj = j - CY(l); sj += (sj < 0) ? P.esj : 0; sj -= (sj >= P.esj) ? P.esj : 0;
                 sk = k - CZ(l); sk += (sk < 0) ? P.esk : 0; sk -= (sk >= P.esk) ? P.esk : 0;
               }
-              double Ux, Uy, Uz;
-              obj_velocity(tab + (long)P.tab_off * SLBD_OBJ_STRIDE, oid[sn], si, sj, sk, P.rho0,
-                           Ux, Uy, Uz);
-              cu = (double)WL(l) * (CX(l) * Ux + CY(l) * Uy + CZ(l) * Uz);
+              const double *po = tab + ((long)P.tab_off + oid[sn]) * SLBD_OBJ_STRIDE;
+              const double rx = si - po[0], ry = sj - po[1], rz = sk - po[2];
+              double cv = 0.0;
+              if (CX(l)) cv += CX(l) * (po[3] + po[7] * rz - po[8] * ry);
+              if (CY(l)) cv += CY(l) * (po[4] + po[8] * rx - po[6] * rz);
+              if (CZ(l)) cv += CZ(l) * (po[5] + po[6] * ry - po[7] * rx);
+              cu = (double)WL(l) * P.uscale * cv;
             }
             if (FORCE && solid) {
               const double a = (double)v[l];
@@ -322,6 +330,7 @@ extern "C" int slbd_batch_commit(slbd_batch *B) {
       q->acc_off = tab_off + r * x->nobj;
       q->moving = x->moving; q->stokes = x->stokes; q->skip_solid = x->skip_solid;
       q->omega = 1.0 / x->tau; q->comega = 1.0 - q->omega; q->rho0 = x->rho0;
+      q->uscale = x->rho0 / 12.0;
       for (int c = 0; c < 3; c++) q->bf[c] = x->bf[c];
       pop_off += 15 * pop_nodes(N);
     }
