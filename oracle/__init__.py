@@ -226,7 +226,9 @@ class RefLib:
             "measure": (None, [SysP]), "accvmean": (None, [SysP, R]),
             "countfluid": (I, [SysP]), "countmask": (I, [SysP, S]), "discrf": (R, [SysP]),
             "newmp": (MPP, [SysP, S]), "mpinit": (None, [MPP]), "mpstep": (None, [MPP]),
-            "mpall": (None, [MPP, R]), "adall": (None, [MPP]), "mprhoinit": (None, [MPP]),
+            "mpall": (None, [MPP, R]),
+            "mpvacfinit": (None, [MPP, MPP, S, C.c_int]), "mpvacf": (None, [MPP, MPP]),
+            "mpvacfwr": (R, [MPP, MPP, S, C.c_int]), "adall": (None, [MPP]), "mprhoinit": (None, [MPP]),
             "adtotal": (R, [MPP, S, I]), "lbloadvel": (None, [SysP, S, S]),
             "lbsavevel": (None, [SysP, S, S]), "lbsave": (None, [SysP, S]),
             "lbload": (None, [SysP, S]), "set_verbosity": (None, [I]),

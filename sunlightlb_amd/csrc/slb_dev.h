@@ -112,6 +112,11 @@ void slbd_mp_step(slbd_lattice *L, slbd_mp *M, double rho0, double alpha, int mo
 /* out: [0] sum fluid P, [1] sum solid P, [2] sum adsorbing-site P,
  *      [3] sum fluid m1, [4] sum fluid m2, [5] fluid count */
 void slbd_mp_sums(slbd_lattice *L, slbd_mp *M, double out[6]);
+/* mpvacf on the device: P -= psi * (sum psi P / sum psi^2) on fluid nodes of
+ * M, then out[0] = sum psi P u_comp (u = j/rho, frozen MP rho), out[1] =
+ * sum psi P2 (P2 of M2), fluid nodes, FP64; psi_host may be NULL */
+void slbd_vacf(slbd_lattice *L, slbd_mp *M, slbd_mp *M2, const double *psi_host, int comp,
+               double out[2]);
 
 /* Timing support: run nsteps fused steps between HIP events on the
  * lattice stream; returns the elapsed milliseconds. */

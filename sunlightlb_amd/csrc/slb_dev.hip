@@ -1514,6 +1514,89 @@ extern "C" void slbd_mp_sums(slbd_lattice *L, slbd_mp *M, double out[6]) {
   copy_out(out, L->dout, sizeof(double) * 6, L->stream);
 }
 
+/* ---- velocity autocorrelation (src/mpvacf.c mpvacf) ---- */
+
+/* pass 1: sum over fluid nodes of psi P and psi^2 */
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_vacf1(const T *__restrict__ q, int W, const uint16_t *__restrict__ flags,
+        const double *__restrict__ psi, Geom G, double *__restrict__ part) {
+  double s[2] = {0.0, 0.0};
+  for (long x = (long)blockIdx.x * blockDim.x + threadIdx.x; x < G.N;
+       x += (long)gridDim.x * blockDim.x) {
+    if (flags[x] & FLAG_SOLID) continue;
+    const double w = psi ? psi[x] : 1.0;
+    s[0] += w * (double)q[x * W];
+    s[1] += w * w;
+  }
+  block_reduce_store<2>(s, part);
+}
+
+/* pass 2: P -= psi pmean on fluid nodes, then the sums of psi P u_b and
+ * psi P2 (u = j/rho with the frozen MP density) */
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_vacf2(const T *__restrict__ a, T *__restrict__ q, int W, const T *__restrict__ q2, int W2,
+        const T *__restrict__ rho, const uint16_t *__restrict__ flags,
+        const double *__restrict__ psi, double pmean, int comp, Geom G,
+        double *__restrict__ part) {
+  double s[2] = {0.0, 0.0};
+  for (long x = (long)blockIdx.x * blockDim.x + threadIdx.x; x < G.N;
+       x += (long)gridDim.x * blockDim.x) {
+    if (flags[x] & FLAG_SOLID) continue;
+    const double w = psi ? psi[x] : 1.0;
+    const T pv = (T)((double)q[x * W] - w * pmean);
+    q[x * W] = pv;
+    double j = 0.0;
+#pragma unroll
+    for (int l = 1; l < 15; l++) {
+      const int c = comp == 0 ? CX(l) : (comp == 1 ? CY(l) : CZ(l));
+      if (c) j += c * (double)a[pidx(l, x)];
+    }
+    const double u = j / (double)rho[x];
+    s[0] += w * (double)pv * u;
+    s[1] += w * (double)q2[x * W2];
+  }
+  block_reduce_store<2>(s, part);
+}
+
+template <typename T>
+static void launch_vacf(slbd_lattice *L, slbd_mp *M, slbd_mp *M2, const double *psi, int comp,
+                        double out[2]) {
+  const int nb = red_blocks(L->N);
+  hipLaunchKernelGGL(k_vacf1<T>, dim3(nb), dim3(256), 0, L->stream, (const T *)M->q[M->cur], M->W,
+                     L->flags, psi, L->G, L->part);
+  LAUNCHCHECK();
+  hipLaunchKernelGGL(k_final<2>, dim3(1), dim3(256), 0, L->stream, L->part, nb, L->dout);
+  LAUNCHCHECK();
+  double h[2];
+  copy_out(h, L->dout, sizeof(double) * 2, L->stream);
+  const double pmean = h[0] / h[1];
+  hipLaunchKernelGGL(k_vacf2<T>, dim3(nb), dim3(256), 0, L->stream,
+                     (const T *)L->pops[L->cur], (T *)M->q[M->cur], M->W,
+                     (const T *)M2->q[M2->cur], M2->W, (const T *)L->rho, L->flags, psi, pmean,
+                     comp, L->G, L->part);
+  LAUNCHCHECK();
+  hipLaunchKernelGGL(k_final<2>, dim3(1), dim3(256), 0, L->stream, L->part, nb, L->dout);
+  LAUNCHCHECK();
+  copy_out(out, L->dout, sizeof(double) * 2, L->stream);
+}
+
+extern "C" void slbd_vacf(slbd_lattice *L, slbd_mp *M, slbd_mp *M2, const double *psi_host,
+                          int comp, double out[2]) {
+  check_mp(L, M);
+  check_mp(L, M2);
+  ensure_mp_lattice(L);
+  const double *psi = NULL;
+  if (psi_host) {
+    if (!L->psi) L->psi = (double *)dalloc(sizeof(double) * L->N);
+    copy_in(L->psi, psi_host, sizeof(double) * L->N, L->stream);
+    psi = L->psi;
+  }
+  if (L->prec == 32) launch_vacf<float>(L, M, M2, psi, comp, out);
+  else launch_vacf<double>(L, M, M2, psi, comp, out);
+}
+
 /* ---- device copy bandwidth (roofline and counter calibration) ---- */
 
 __global__ void __launch_bounds__(256)

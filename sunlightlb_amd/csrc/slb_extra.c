@@ -1,8 +1,8 @@
 /*
- * slb_extra.c -- velocity-autocorrelation and image entry points of the
- * sunlightlb.h API (reference src/mpvacf.c, src/mpimage.c).  These analysis
- * and visualisation routines are outside the GPU hot path; here they report
- * that they are not provided by this library and return neutral values.
+ * slb_extra.c -- image entry points of the sunlightlb.h API (reference
+ * src/mpimage.c).  These visualisation routines are outside the GPU hot
+ * path; here they report that they are not provided by this library and
+ * return neutral values.  (The velocity autocorrelation is in slb_vacf.c.)
  */
 #include <stdio.h>
 #include <stdlib.h>
@@ -12,10 +12,6 @@ static void unsupported(const char *what) {
   fprintf(stderr, "sunlightlb: %s is not provided by the MI355X library\n", what);
 }
 
-void mpvacfinit(MPSys mp, MPSys mp2, char *vacfdir, int n) { (void)mp; (void)mp2; (void)vacfdir; (void)n; unsupported("mpvacfinit"); }
-void mpvacfinit2(MPSys mp, MPSys mp2) { (void)mp; (void)mp2; unsupported("mpvacfinit2"); }
-void mpvacf(MPSys mp, MPSys mp2) { (void)mp; (void)mp2; unsupported("mpvacf"); }
-LBReal mpvacfwr(MPSys mp, MPSys mp2, char *f, int n) { (void)mp; (void)mp2; (void)f; (void)n; unsupported("mpvacfwr"); return 0.0; }
 
 LBColour newlbcolour(int r, int g, int b) {
   LBColour c = (LBColour)malloc(sizeof(LBColour_type));

@@ -247,6 +247,50 @@ def case_loadvel(R):
     print("wrote loadvel")
 
 
+def case_vacf(R):
+    """Tagged-tracer velocity autocorrelation (src/mpvacf.c) on the flow round
+    a sphere: P and P2 after mpvacfinit, and the diffusion coefficient
+    returned by mpvacfwr right after the initialisation and after 12
+    (mpstep, mpstep, mpvacf) rounds, for the directions xx and xy."""
+    import tempfile
+    dims = (12, 10, 14)
+    out = {}
+    h = R.new(*dims)
+    s = h.contents
+    s.tau, s.stokes_flow, s.skip_solid = 0.8, 1, 0
+    R.L.insertsph(h, 3.1, 5.2, 4.7, 6.9)
+    R.L.zerobf(h)
+    R.L.addbf(h, 2e-4, 5e-5, 0.0)
+    for _ in range(40):
+        R.L.lbstep(h)
+    N = s.nnodes
+    st = state(R, h)
+    out["a"], out["mask"], out["objid"] = st["a"], st["mask"], st["objid"]
+    mp = R.L.newmp(h, b"null")
+    mp2 = R.L.newmp(h, b"null")
+    mp.contents.alpha = mp2.contents.alpha = 0.1
+    R.L.mprhoinit(mp)
+    out["rho"] = R.arr(s.rho, N).astype(np.float64).copy()
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "vacf.txt").encode()
+        for dr in ("xx", "xy"):
+            R.L.mpinit(mp)
+            R.L.mpinit(mp2)
+            R.L.mpvacfinit(mp, mp2, dr.encode(), 40)
+            out[f"{dr}_p"] = R.arr(mp.contents.p, N).astype(np.float64).copy()
+            out[f"{dr}_p2"] = R.arr(mp2.contents.p, N).astype(np.float64).copy()
+            out[f"{dr}_d0"] = np.array(R.L.mpvacfwr(mp, mp2, path, 1))
+            for _ in range(12):
+                R.L.mpstep(mp)
+                R.L.mpstep(mp2)
+                R.L.mpvacf(mp, mp2)
+            out[f"{dr}_d12"] = np.array(R.L.mpvacfwr(mp, mp2, path, 1))
+    out["meta"] = np.array(json.dumps(dict(dims=dims, alpha=0.1, tau=0.8, steps=40, rounds=12,
+                                           sphere=[3.1, 5.2, 4.7, 6.9], bf=[2e-4, 5e-5, 0.0])))
+    np.savez_compressed(os.path.join(OUT, "vacf.npz"), **out)
+    print("wrote vacf")
+
+
 def main():
     R = RefLib(64)
     R32 = RefLib(32)
@@ -260,6 +304,7 @@ def main():
     case_mp(R)
     case_loadvel(R)
     case_config1(R, R32)
+    case_vacf(R)
 
 
 if __name__ == "__main__":
