@@ -476,12 +476,6 @@ def warm_workload(L, args):
                 f"{wt['steps_to_1e-05']}, pred err u/rho/s "
                 + "/".join(f"{x:.2e}" for x in row["pred_rel_mse_groups"]))
             rows.append(row)
-            log(f"warm eval n={size} R={p[0]:.2f} phi={phi:.3f}: cold {row['cold']['steps']} steps "
-                f"{row['cold']['seconds']:.3f} s, warm {row['warm']['steps']} steps "
-                f"{row['warm']['seconds']:.3f} s (used {row['warm']['used_warm']}), "
-                f"dk/k {row['warm']['k_rel_diff']:.1e}, pred err u/rho/s "
-                + "/".join(f"{x:.2e}" for x in row["pred_rel_mse_groups"])
-                + (f", equilibrium {row['equilibrium']['steps']} steps" if "equilibrium" in row else ""))
     L.slb_warm_free(net)
     summ = {}
     for size in sizes:

@@ -1688,19 +1688,23 @@ __global__ void k_jfa_pass(const int *__restrict__ in, int *__restrict__ out, in
 __global__ void __launch_bounds__(256)
 k_fluid_dist(const uint16_t *__restrict__ flags, const int *__restrict__ ns, double *__restrict__ out,
              Geom G) {
-  const long n = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  __shared__ double sm[2][4];
   double d = 0.0, c = 0.0;
-  if (n < G.N && !(flags[n] & FLAG_SOLID)) {
-    d = (ns[n] >= 0) ? (double)sqrtf((float)pdist2(G, n, ns[n])) : 64.0;
-    c = 1.0;
+  for (long n = (long)blockIdx.x * blockDim.x + threadIdx.x; n < G.N;
+       n += (long)gridDim.x * blockDim.x) {
+    if (flags[n] & FLAG_SOLID) continue;
+    d += (ns[n] >= 0) ? (double)sqrtf((float)pdist2(G, n, ns[n])) : 64.0;
+    c += 1.0;
   }
   for (int o = 32; o > 0; o >>= 1) {
     d += __shfl_xor(d, o, 64);
     c += __shfl_xor(c, o, 64);
   }
-  if ((threadIdx.x & 63) == 0 && c > 0.0) {
-    atomicAdd(out, d);
-    atomicAdd(out + 1, c);
+  if ((threadIdx.x & 63) == 0) { sm[0][threadIdx.x >> 6] = d; sm[1][threadIdx.x >> 6] = c; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    atomicAdd(out, sm[0][0] + sm[0][1] + sm[0][2] + sm[0][3]);
+    atomicAdd(out + 1, sm[1][0] + sm[1][1] + sm[1][2] + sm[1][3]);
   }
 }
 
@@ -1855,8 +1859,8 @@ extern "C" double slbd_features(slbd_lattice *L, uint16_t *X, int C) {
   double *acc, h[2];
   HIPCHECK(hipMalloc(&acc, 2 * sizeof(double)));
   HIPCHECK(hipMemsetAsync(acc, 0, 2 * sizeof(double), L->stream));
-  hipLaunchKernelGGL(k_fluid_dist, lin_grid(L->N), dim3(256), 0, L->stream, L->flags, res[0], acc,
-                     L->G);
+  hipLaunchKernelGGL(k_fluid_dist, dim3((unsigned)red_blocks(L->N)), dim3(256), 0, L->stream,
+                     L->flags, res[0], acc, L->G);
   LAUNCHCHECK();
   HIPCHECK(hipMemcpyAsync(h, acc, 2 * sizeof(double), hipMemcpyDeviceToHost, L->stream));
   HIPCHECK(hipStreamSynchronize(L->stream));

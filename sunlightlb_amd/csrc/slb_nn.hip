@@ -362,15 +362,24 @@ __global__ void k_relu_bwd(const uint16_t *__restrict__ Y, const uint16_t *__res
   dP[e] = (bf2f(Y[e]) > 0.f) ? dY[e] : (uint16_t)0;
 }
 
-/* db[c] += sum_n dP[n][c] */
+/* db[c] += sum_n dP[n][c] for C dividing 256: thread t sums channel t % C
+ * over rows t / C, t / C + 256/C, ... of its block's grid-stride share
+ * (coalesced rows), the block reduces in LDS and adds one value per channel */
 __global__ void __launch_bounds__(256)
 k_bias_grad(const uint16_t *__restrict__ dP, int C, int N, float *__restrict__ db) {
-  const int c = threadIdx.x & 63;
-  const int rowgrp = threadIdx.x >> 6;
-  if (c >= C) return;
+  __shared__ float sm[256];
+  const int R = 256 / C;
+  const int c = threadIdx.x % C, r = threadIdx.x / C;
   float s = 0.f;
-  for (int n = blockIdx.x * 4 + rowgrp; n < N; n += gridDim.x * 4) s += bf2f(dP[(long)n * C + c]);
-  atomicAdd(db + c, s);
+  for (long n = (long)blockIdx.x * R + r; n < N; n += (long)gridDim.x * R)
+    s += bf2f(dP[n * C + c]);
+  sm[threadIdx.x] = s;
+  __syncthreads();
+  if (threadIdx.x < C) {
+    float t = 0.f;
+    for (int k = 0; k < R; k++) t += sm[k * C + threadIdx.x];
+    atomicAdd(db + threadIdx.x, t);
+  }
 }
 
 /* 2x2x2 average pooling (channels-last) and its gradient */
@@ -436,21 +445,24 @@ __global__ void k_add_into(uint16_t *__restrict__ A, const uint16_t *__restrict_
 __global__ void __launch_bounds__(256)
 k_mse(const float *__restrict__ Yf, int C, const uint16_t *__restrict__ T, int Ct, int Cv,
       int N, const float *__restrict__ wc, uint16_t *__restrict__ dY, float *__restrict__ loss) {
-  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  __shared__ float sm[4];
   float s = 0.f;
-  if (n < N) {
+  for (long n = (long)blockIdx.x * blockDim.x + threadIdx.x; n < N;
+       n += (long)gridDim.x * blockDim.x) {
     for (int c = 0; c < C; c++) {
       float g = 0.f;
       if (c < Cv) {
-        const float d = Yf[(long)n * C + c] - bf2f(T[(long)n * Ct + c]);
+        const float d = Yf[n * C + c] - bf2f(T[n * Ct + c]);
         s += wc[c] * d * d;
         g = 2.f * d * wc[c];
       }
-      dY[(long)n * C + c] = f2bf(g);
+      dY[n * C + c] = f2bf(g);
     }
   }
   for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
-  if ((threadIdx.x & 63) == 0) atomicAdd(loss, s);
+  if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) atomicAdd(loss, sm[0] + sm[1] + sm[2] + sm[3]);
 }
 
 /* Adam on FP32 master weights, refreshing the BF16 copy (zero-padded K
@@ -624,11 +636,15 @@ extern "C" void slbn_relu_bwd(const uint16_t *Y, const uint16_t *dY, uint16_t *d
 }
 
 extern "C" void slbn_bias_grad(const uint16_t *dP, int C, int N, float *db) {
-  if (C < 1 || C > 64) {
-    fprintf(stderr, "slbn_bias_grad: 1..64 channels\n");
+  if (C < 1 || C > 256 || 256 % C) {
+    fprintf(stderr, "slbn_bias_grad: the channel count must divide 256\n");
     exit(EXIT_FAILURE);
   }
-  hipLaunchKernelGGL(k_bias_grad, dim3(256), dim3(256), 0, nn_stream, dP, C, N, db);
+  const long rows = 256 / C;
+  long nb = ((long)N + rows - 1) / rows;
+  if (nb > 1024) nb = 1024;
+  if (nb < 1) nb = 1;
+  hipLaunchKernelGGL(k_bias_grad, dim3((unsigned)nb), dim3(256), 0, nn_stream, dP, C, N, db);
   LAUNCHCHECK();
 }
 
@@ -663,8 +679,9 @@ extern "C" void slbn_add_into(uint16_t *A, const uint16_t *B, long n) {
 }
 extern "C" void slbn_mse(const float *Yf, int C, const uint16_t *T, int Ct, int Cv, int N,
                          const float *wc, uint16_t *dY, float *loss) {
-  hipLaunchKernelGGL(k_mse, dim3(blocks_for(N, 256)), dim3(256), 0, nn_stream, Yf, C, T, Ct, Cv, N,
-                     wc, dY, loss);
+  unsigned nb = blocks_for(N, 256);
+  if (nb > 2048) nb = 2048;
+  hipLaunchKernelGGL(k_mse, dim3(nb), dim3(256), 0, nn_stream, Yf, C, T, Ct, Cv, N, wc, dY, loss);
   LAUNCHCHECK();
 }
 extern "C" void slbn_adam(float *w, uint16_t *wb, const float *g, float *m, float *v, long n,

@@ -31,9 +31,8 @@ def mp_field(lat, mp):
 
 
 def test_vacf_matches_reference():
-    import json
-    g = load_golden("vacf")
-    m = json.loads(str(g["meta"]))
+    g = load_golden("vacf.npz")
+    m = g["meta"]
     lat = lattice(tuple(m["dims"]))
     L, h = lat.L, lat.h
     lat.s.tau, lat.s.stokes_flow, lat.s.skip_solid = m["tau"], 1, 0
