@@ -622,7 +622,8 @@ def main():
                         f"{args.radius:g} spheres to lattice solid fraction {args.phi:g} "
                         f"(seed {args.seed}), body force {args.force:g} x, Stokes, tau 0.75",
             "nodes": N, "spheres": nsph, "solid_fraction": round(phi, 5),
-            "populations": f"{dtype} SoA shifted (a = f - w rho0/72), pull, ping-pong",
+            "populations": f"{dtype} shifted (a = f - w rho0/72), tiles of 1024 nodes x 15 runs, "
+                           "pull, ping-pong",
         },
         "roofline": {
             "bound": "hbm",

@@ -477,7 +477,12 @@ extern "C" int slbd_batch_box_dims(const slbd_batch *B, int box, int *dims) {
   return B->box[box].nrhs;
 }
 extern "C" void slbd_batch_get_config(const slbd_batch *B, int *cfg) {
-  cfg[0] = B->block; cfg[1] = B->npt; cfg[2] = B->grid; cfg[3] = B->ntiles;
+  /* tile count of the configured shape (the table itself is rebuilt lazily
+   * at the next run, so B->ntiles may still describe an earlier shape) */
+  const long per = (long)B->block * B->npt;
+  long nt = 0;
+  for (int p = 0; p < B->nprob; p++) nt += (B->hprobs[p].N + per - 1) / per;
+  cfg[0] = B->block; cfg[1] = B->npt; cfg[2] = B->grid; cfg[3] = (int)nt;
 }
 extern "C" int slbd_batch_cu_count(void) {
   hipDeviceProp_t prop;
