@@ -415,6 +415,24 @@ def warm_workload(L, args):
                 "ms_per_iter": round(dt / args.warm_iters * 1e3, 3), "loss_curve": curve,
                 "final_loss": last, "fallback_probe_steps": pr.value,
                 "fallback_probe_residual": pres.value})
+    # ---- start-policy calibration: warm solves of unseen training-distribution
+    # packs, without fallback, for their median residual curve ----
+    t0 = time.perf_counter()
+    crng = np.random.default_rng(args.seed + 13)
+    calib = []
+    for solid, phi in warm_maps(L, n, warm_params(crng, args.warm_calib), workers):
+        h = warm_lattice(L, n, solid, F)
+        L.slb_warm_set_mode(net, 1)
+        L.slb_warm_start(net, h)
+        steps, hist, r = solve_loop(L, h, args.tol, args.interval, args.maxsteps)
+        L.slb_warm_note_warm(net, (C.c_double * len(hist))(*hist), len(hist), args.interval)
+        calib.append(steps)
+        L.slb_destroy(h)
+    res["calibration"] = {"packs": args.warm_calib, "seconds": round(time.perf_counter() - t0, 1),
+                          "warm_steps_median": float(np.median(calib)) if calib else None,
+                          "start_warm_at": {f"{tol:.0e}": L.slb_warm_policy(net, tol, args.interval)
+                                            for tol in (1e-3, 1e-4, 1e-5, 1e-6)}}
+    log(f"warm: start policy {res['calibration']['start_warm_at']}")
     if args.warm_save:
         L.slb_warm_save(net, args.warm_save.encode())
     # ---- evaluation on unseen packs ----
@@ -501,7 +519,7 @@ def warm_workload(L, args):
             "cold_seconds_total": float(cs.sum()), "warm_seconds_total": float(ws.sum()),
             "wall_time_ratio_cold_over_warm": float(cs.sum() / ws.sum()),
             "steps_ratio_cold_over_warm": float(cst.sum() / wst.sum()),
-            "fell_back_to_cold": int(sum(1 for r in rs if r["warm"]["used_warm"] == 0)),
+            "started_or_fell_back_cold": int(sum(1 for r in rs if r["warm"]["used_warm"] == 0)),
             "k_rel_diff_max": float(dk.max()), "k_rel_diff_median": float(np.median(dk)),
             "no_fallback_k_rel_diff_max": float(dkt.max()),
             "no_fallback_steps_ratio": {t: ratio(f"steps_to_{t}", "warm_trace")
@@ -552,6 +570,8 @@ def main():
     ap.add_argument("--warm-iters", type=int, default=8000)
     ap.add_argument("--warm-lr", type=float, default=1e-3)
     ap.add_argument("--warm-eval", type=int, default=64)
+    ap.add_argument("--warm-calib", type=int, default=16,
+                    help="unseen packs solved warm after training for the start policy")
     ap.add_argument("--warm-eval-sizes", default="128,256")
     ap.add_argument("--warm-save", default="")
     ap.add_argument("--copy-test", action="store_true",

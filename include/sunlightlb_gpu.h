@@ -252,9 +252,19 @@ void   slb_warm_start(slb_warmnet *net, LBSys lb);
 void   slb_warm_set_fallback(slb_warmnet *net, int probe, double probe_res);
 void   slb_warm_get_fallback(const slb_warmnet *net, int *probe, double *probe_res);
 void   slb_warm_note_cold(slb_warmnet *net, const double *res, int n, int interval);
-/* Warm start, then run to slb_residual() < tol as slb_run_to_residual does,
- * with the fallback.  Returns the total steps; *used_warm is 0 when the
- * solve fell back to a cold start. */
+/* Start policy.  note_warm records the residual history of a calibration
+ * warm solve (slb_warm_start, then plain steps; same form as note_cold); the
+ * network keeps the median curve.  slb_warm_policy returns 1 when a solve to
+ * tol should start warm: always, unless both median curves exist for this
+ * interval and the warm one falls below tol no sooner than the cold one.
+ * slb_warm_solve starts cold (from rest) when it returns 0.  set_fallback
+ * clears both curves. */
+void   slb_warm_note_warm(slb_warmnet *net, const double *res, int n, int interval);
+int    slb_warm_policy(const slb_warmnet *net, double tol, int interval);
+/* Warm start (or a cold one, by the start policy), then run to
+ * slb_residual() < tol as slb_run_to_residual does, with the fallback.
+ * Returns the total steps; *used_warm is 0 when the solve started cold or
+ * fell back to a cold start. */
 int    slb_warm_solve(slb_warmnet *net, LBSys lb, double tol, int interval, int maxsteps,
                       double *res, int *used_warm);
 /* Parameter access for tests: the layer table, the parameter arena

@@ -193,6 +193,8 @@ SIGNATURES = {
     "slb_warm_set_fallback": (None, [C.c_void_p, I, D]),
     "slb_warm_get_fallback": (None, [C.c_void_p, IntP, RealP]),
     "slb_warm_note_cold": (None, [C.c_void_p, RealP, I, I]),
+    "slb_warm_note_warm": (None, [C.c_void_p, RealP, I, I]),
+    "slb_warm_policy": (I, [C.c_void_p, D, I]),
     "slb_warm_solve": (I, [C.c_void_p, LBSys, D, I, I, RealP, IntP]),
     "slb_warm_nparams": (C.c_long, [C.c_void_p]),
     "slb_warm_layout": (None, [IntP, IntP, IntP, LongP, LongP]),

@@ -34,18 +34,24 @@ struct BProb {
   int acc_off;    /* first object accumulator (6 doubles each) */
   int moving, stokes, skip_solid;
   double omega, comega, rho0;
-  double uscale;  /* rho0 / 12: wall velocity scale of vspread */
   double bf[3];
 };
 
 #define BATCH_LDS_OBJ 64
 
+/* Wall rows: per object row, in the working precision and scaled by
+ * rho0/12, w0 = v - omega x pos (the velocity of the lattice origin in the
+ * object's rigid motion) and omega, so that the wall velocity at solid node s
+ * is U = w0 + omega x s and only c.U = c.w0 + c.(omega x s) enters a link. */
+#define WROW 8
+
 template <typename T, bool FORCE, int NPT>
 __global__ void __launch_bounds__(512)
 k_bstep(const T *__restrict__ src, T *__restrict__ dst,
-                        const uint16_t *__restrict__ flags, const int32_t *__restrict__ objidx,
-                        const double *__restrict__ tab, const BProb *__restrict__ probs,
-                        const int2 *__restrict__ tiles, int ntiles, double *__restrict__ acc) {
+        const uint16_t *__restrict__ flags, const int32_t *__restrict__ objidx,
+        const double *__restrict__ tab, const T *__restrict__ wtab,
+        const BProb *__restrict__ probs,
+        const int2 *__restrict__ tiles, int ntiles, double *__restrict__ acc) {
   __shared__ double sacc[FORCE ? BATCH_LDS_OBJ * 6 : 1];
   for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
     const int2 td = tiles[tile];
@@ -66,24 +72,24 @@ k_bstep(const T *__restrict__ src, T *__restrict__ dst,
     const bool bfon = (P.bf[0] != 0.0 || P.bf[1] != 0.0 || P.bf[2] != 0.0);
 #pragma unroll
     for (int q = 0; q < NPT; q++) {
-      const long x = (long)td.y + (long)q * blockDim.x + threadIdx.x;
+      /* 15 pop_nodes(N) < 2^31 per problem (checked at commit): 32-bit
+       * node and population indices */
+      const int x = td.y + q * (int)blockDim.x + (int)threadIdx.x;
       if (x >= P.N) continue;
-      /* N < 2^31 per problem: 32-bit division */
-      const int xi = (int)x;
-      const int i = xi / P.plane;
-      const int rem = xi - i * P.plane;
+      const int i = x / P.plane;
+      const int rem = x - i * P.plane;
       const int j = rem / P.esk, k = rem - j * P.esk;
-      const AxisOff oi = axis_off(i, P.esi, P.plane);
-      const AxisOff oj = axis_off(j, P.esj, P.esk);
-      const AxisOff ok = axis_off(k, P.esk, 1);
+      const AxisOff32 oi = axis_off32(i, P.esi, P.plane);
+      const AxisOff32 oj = axis_off32(j, P.esj, P.esk);
+      const AxisOff32 ok = axis_off32(k, P.esk, 1);
       const unsigned f = fl[x];
       const bool solid = (f & FLAG_SOLID) != 0;
       T v[15];
-      v[0] = s[pidx(0, x)];
+      v[0] = s[pidx32(0, x)];
       Links<1, 15>::run([&](auto LL) {
         constexpr int l = decltype(LL)::value;
         const bool bb = (f >> (l - 1)) & 1u;
-        const long idx = bb ? pidx(OPP(l), x) : pidx(l, x + nb_off<l, -1>(oi, oj, ok));
+        const int idx = bb ? pidx32(OPP(l), x) : pidx32(l, x + nb_off32<l, -1>(oi, oj, ok));
         v[l] = s[idx];
       });
       if ((P.moving || FORCE) && (f & FLAG_LINKS)) {
@@ -91,14 +97,14 @@ k_bstep(const T *__restrict__ src, T *__restrict__ dst,
         Links<1, 15>::run([&](auto LL) {
           constexpr int l = decltype(LL)::value;
           if ((f >> (l - 1)) & 1u) {
-            const long xs = x + nb_off<l, -1>(oi, oj, ok);
-            double cu = 0.0;
+            const int xs = x + nb_off32<l, -1>(oi, oj, ok);
+            T cu = T(0);
             if (P.moving) {
               /* wall velocity U = (rho0/12)(v + w x r) of the solid end of
-               * the link (vspread, src/lbobjs.c:565-589, r without minimum
-               * image); only its component along c_l enters:
-               * c.U = (rho0/12)(c.v + c.(w x r)) */
-              long sn = x;
+               * the link (vspread, src/lbobjs.c:565-589, r = s - pos without
+               * minimum image) = w0 + w x s with the scaled wall row; only
+               * its component along c_l enters */
+              int sn = x;
               int si = i, sj = j, sk = k;
               if (!solid) {
                 sn = xs;
@@ -106,23 +112,23 @@ k_bstep(const T *__restrict__ src, T *__restrict__ dst,
                 sj = j - CY(l); sj += (sj < 0) ? P.esj : 0; sj -= (sj >= P.esj) ? P.esj : 0;
                 sk = k - CZ(l); sk += (sk < 0) ? P.esk : 0; sk -= (sk >= P.esk) ? P.esk : 0;
               }
-              const double *po = tab + ((long)P.tab_off + oid[sn]) * SLBD_OBJ_STRIDE;
-              const double rx = si - po[0], ry = sj - po[1], rz = sk - po[2];
-              double cv = 0.0;
-              if (CX(l)) cv += CX(l) * (po[3] + po[7] * rz - po[8] * ry);
-              if (CY(l)) cv += CY(l) * (po[4] + po[8] * rx - po[6] * rz);
-              if (CZ(l)) cv += CZ(l) * (po[5] + po[6] * ry - po[7] * rx);
-              cu = (double)WL(l) * P.uscale * cv;
+              const T *wr = wtab + ((long)P.tab_off + oid[sn]) * WROW;
+              const T sx = T(si), sy = T(sj), sz = T(sk);
+              T cv = T(0);
+              if (CX(l)) cv += T(CX(l)) * (wr[0] + wr[4] * sz - wr[5] * sy);
+              if (CY(l)) cv += T(CY(l)) * (wr[1] + wr[5] * sx - wr[3] * sz);
+              if (CZ(l)) cv += T(CZ(l)) * (wr[2] + wr[3] * sy - wr[4] * sx);
+              cu = T(WL(l)) * cv;
             }
             if (FORCE && solid) {
               const double a = (double)v[l];
-              const double b = (double)s[pidx(l, xs)];
-              const double g = -2.0 * (a - b + cu);
+              const double b = (double)s[pidx32(l, xs)];
+              const double g = -2.0 * (a - b + (double)cu);
               if (CX(l)) Fx += CX(l) * g;
               if (CY(l)) Fy += CY(l) * g;
               if (CZ(l)) Fz += CZ(l) * g;
             }
-            if (P.moving) v[l] = (T)((double)v[l] + cu);
+            if (P.moving) v[l] += cu;
           }
         });
         if (FORCE && solid && (Fx != 0.0 || Fy != 0.0 || Fz != 0.0)) {
@@ -141,7 +147,7 @@ k_bstep(const T *__restrict__ src, T *__restrict__ dst,
         collide_node<T>(v, jbx, jby, jbz, A);
       }
 #pragma unroll
-      for (int l = 0; l < 15; l++) d[pidx(l, x)] = v[l];
+      for (int l = 0; l < 15; l++) d[pidx32(l, x)] = v[l];
     }
     if (lds) {
       __syncthreads();
@@ -200,6 +206,7 @@ struct slbd_batch {
   uint16_t *flags;
   int32_t *objidx;
   double *tab, *acc;
+  void *wtab;               /* wall rows in the working precision */
   BProb *probs;
   int2 *tiles;
   int nprob, ntiles, nacc;
@@ -236,6 +243,11 @@ extern "C" int slbd_batch_add_box(slbd_batch *B, int esi, int esj, int esk,
                                   int nrhs, const double *tabs, double tau, double rho0,
                                   int stokes, int skip_solid, int moving, const double *bf) {
   if (B->committed || nrhs < 1) return -1;
+  if (15 * pop_nodes((long)esi * esj * esk) > 0x7fffffffL) {
+    fprintf(stderr, "slb_batch_add: a %d x %d x %d box is too large for a batch (at most "
+            "2^31 populations per box)\n", esi, esj, esk);
+    return -1;
+  }
   if (B->nbox == B->cap) {
     B->cap = B->cap ? 2 * B->cap : 64;
     B->box = (BBox *)realloc(B->box, sizeof(BBox) * B->cap);
@@ -301,6 +313,8 @@ extern "C" int slbd_batch_commit(slbd_batch *B) {
   int32_t *dmask;
   HIPCHECK(hipMalloc(&dmask, sizeof(int32_t) * nodes));
   HIPCHECK(hipMalloc(&B->tab, sizeof(double) * SLBD_OBJ_STRIDE * (ntab > 0 ? ntab : 1)));
+  HIPCHECK(hipMalloc(&B->wtab, bts(B->prec) * WROW * (ntab > 0 ? ntab : 1)));
+  double *hw = (double *)calloc((size_t)WROW * (ntab > 0 ? ntab : 1), sizeof(double));
   HIPCHECK(hipMalloc(&B->acc, sizeof(double) * 6 * (nacc > 0 ? nacc : 1)));
   HIPCHECK(hipMemsetAsync(B->acc, 0, sizeof(double) * 6 * (nacc > 0 ? nacc : 1), B->stream));
   long node_off = 0, pop_off = 0;
@@ -314,6 +328,15 @@ extern "C" int slbd_batch_commit(slbd_batch *B) {
       HIPCHECK(hipMemcpy(B->tab + (long)tab_off * SLBD_OBJ_STRIDE, x->tab,
                          sizeof(double) * SLBD_OBJ_STRIDE * x->nobj * x->nrhs,
                          hipMemcpyHostToDevice));
+    for (long o = 0; o < (long)x->nobj * x->nrhs; o++) {
+      const double *po = x->tab + o * SLBD_OBJ_STRIDE;   /* pos, vel, ang */
+      double *w = hw + (tab_off + o) * WROW;
+      const double us = x->rho0 / 12.0;
+      w[0] = us * (po[3] - (po[7] * po[2] - po[8] * po[1]));
+      w[1] = us * (po[4] - (po[8] * po[0] - po[6] * po[2]));
+      w[2] = us * (po[5] - (po[6] * po[1] - po[7] * po[0]));
+      w[3] = us * po[6]; w[4] = us * po[7]; w[5] = us * po[8];
+    }
     hipLaunchKernelGGL(k_bflags, dim3((x->esj * x->esk + 255) / 256, x->esi), dim3(256), 0,
                        B->stream, B->objidx + node_off, dmask + node_off, B->flags + node_off,
                        x->esi, x->esj, x->esk);
@@ -330,7 +353,6 @@ extern "C" int slbd_batch_commit(slbd_batch *B) {
       q->acc_off = tab_off + r * x->nobj;
       q->moving = x->moving; q->stokes = x->stokes; q->skip_solid = x->skip_solid;
       q->omega = 1.0 / x->tau; q->comega = 1.0 - q->omega; q->rho0 = x->rho0;
-      q->uscale = x->rho0 / 12.0;
       for (int c = 0; c < 3; c++) q->bf[c] = x->bf[c];
       pop_off += 15 * pop_nodes(N);
     }
@@ -339,6 +361,17 @@ extern "C" int slbd_batch_commit(slbd_batch *B) {
     free(x->objidx); free(x->mask); free(x->tab);
     x->objidx = x->mask = NULL; x->tab = NULL;
   }
+  if (B->prec == 32) {
+    float *hf = (float *)malloc(sizeof(float) * WROW * (ntab > 0 ? ntab : 1));
+    for (long q = 0; q < (long)WROW * (ntab > 0 ? ntab : 1); q++) hf[q] = (float)hw[q];
+    HIPCHECK(hipMemcpy(B->wtab, hf, sizeof(float) * WROW * (ntab > 0 ? ntab : 1),
+                       hipMemcpyHostToDevice));
+    free(hf);
+  } else {
+    HIPCHECK(hipMemcpy(B->wtab, hw, sizeof(double) * WROW * (ntab > 0 ? ntab : 1),
+                       hipMemcpyHostToDevice));
+  }
+  free(hw);
   HIPCHECK(hipStreamSynchronize(B->stream));
   HIPCHECK(hipFree(dmask));
   HIPCHECK(hipMalloc(&B->probs, sizeof(BProb) * nprob));
@@ -356,7 +389,7 @@ static void blaunch_t(slbd_batch *B) {
   const int grid = (B->grid > 0 && B->grid < B->ntiles) ? B->grid : B->ntiles;
   hipLaunchKernelGGL((k_bstep<T, FORCE, NPT>), dim3(grid), dim3(B->block), 0, B->stream,
                      (const T *)B->pops[B->cur], (T *)B->pops[B->cur ^ 1], B->flags, B->objidx,
-                     B->tab, B->probs, B->tiles, B->ntiles, B->acc);
+                     B->tab, (const T *)B->wtab, B->probs, B->tiles, B->ntiles, B->acc);
   LAUNCHCHECK();
   B->cur ^= 1;
 }
@@ -494,7 +527,8 @@ extern "C" void slbd_batch_destroy(slbd_batch *B) {
   if (!B) return;
   HIPCHECK(hipStreamSynchronize(B->stream));
   if (B->graph_valid) HIPCHECK(hipGraphExecDestroy(B->gexec));
-  void *ps[] = {B->pops[0], B->pops[1], B->flags, B->objidx, B->tab, B->acc, B->probs, B->tiles};
+  void *ps[] = {B->pops[0], B->pops[1], B->flags, B->objidx, B->tab, B->wtab, B->acc, B->probs,
+                B->tiles};
   for (size_t q = 0; q < sizeof ps / sizeof ps[0]; q++)
     if (ps[q]) HIPCHECK(hipFree(ps[q]));
   for (int b = 0; b < B->nbox; b++) { free(B->box[b].objidx); free(B->box[b].mask); free(B->box[b].tab); }

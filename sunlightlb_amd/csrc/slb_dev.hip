@@ -580,20 +580,30 @@ k_mpstep(const T *__restrict__ a, const T *__restrict__ qin, T *__restrict__ qou
     np += pr * qx[0];
     if (NMOM > 1) { n1 += pr * qx[1]; n2 += pr * qx[2]; }
   }
+  /* Branch-free gather: every source y = x - c_l is an in-range node
+   * (periodic wrap), so its record and population are loaded whether or not
+   * the link carries tracer, all 28 loads are issued before any is used, and
+   * a select drops the links from solid sources. */
+  V qv[15];
+  T av[15];
   Links<1, 15>::run([&](auto LL) {
     constexpr int l = decltype(LL)::value;
-    if (f & (1u << l)) {
-      const long y = x + nb_off<l, -1>(oi, oj, ok);
-      const V qy = Q[y];
-      const T ri = qy[W - 1], P = qy[0];
-      const T pr = a[pidx(l, y)] * ri + T(WL(l) / 72.0) * (r0 * ri - al);
-      np += pr * P;
-      if (NMOM > 1) {
-        const int dd = (A.axis == 0) ? CX(l) : (A.axis == 1) ? CY(l) : CZ(l);
-        const T q1 = qy[1];
-        n1 += pr * (q1 + T(dd) * P);
-        n2 += pr * (qy[2] + T(2 * dd) * q1 + T(dd * dd) * P);
-      }
+    const long y = x + nb_off<l, -1>(oi, oj, ok);
+    qv[l] = Q[y];
+    av[l] = a[pidx(l, y)];
+  });
+  Links<1, 15>::run([&](auto LL) {
+    constexpr int l = decltype(LL)::value;
+    const V qy = qv[l];
+    const T ri = qy[W - 1], P = qy[0];
+    const T pr0 = av[l] * ri + T(WL(l) / 72.0) * (r0 * ri - al);
+    const T pr = (f & (1u << l)) ? pr0 : T(0);
+    np += pr * P;
+    if (NMOM > 1) {
+      const int dd = (A.axis == 0) ? CX(l) : (A.axis == 1) ? CY(l) : CZ(l);
+      const T q1 = qy[1];
+      n1 += pr * (q1 + T(dd) * P);
+      n2 += pr * (qy[2] + T(2 * dd) * q1 + T(dd * dd) * P);
     }
   });
   if (f >> 17) {
@@ -601,8 +611,12 @@ k_mpstep(const T *__restrict__ a, const T *__restrict__ qin, T *__restrict__ qou
     const T keep = T(1.0 - A.pads);
     Links<1, 15>::run([&](auto LL) {
       constexpr int l = decltype(LL)::value;
+      av[l] = a[pidx(OPP(l), x)];
+    });
+    Links<1, 15>::run([&](auto LL) {
+      constexpr int l = decltype(LL)::value;
       if (f & (1u << (16 + l))) {
-        T pr = a[pidx(OPP(l), x)] * rix + T(WL(l) / 72.0) * (r0 * rix - al);
+        T pr = av[l] * rix + T(WL(l) / 72.0) * (r0 * rix - al);
         if (MOVING) {
           const long s = x + nb_off<l, -1>(oi, oj, ok);
           const int si = (i - CX(l) + G.esi) % G.esi;

@@ -92,6 +92,31 @@ __device__ __forceinline__ long nb_off(const AxisOff &oi, const AxisOff &oj, con
   return o;
 }
 
+/* 32-bit forms of the above, for lattices of fewer than 2^31 / 15
+ * populations (the batched boxes): half the address registers. */
+__host__ __device__ __forceinline__ int pidx32(int l, int x) {
+  return ((x >> POP_SHIFT) * 15 + l) * (int)POP_TILE + (x & (int)(POP_TILE - 1));
+}
+struct AxisOff32 {
+  int m, p;
+};
+__device__ __forceinline__ AxisOff32 axis_off32(int q, int n, int st) {
+  AxisOff32 o;
+  o.m = (q > 0) ? -st : (n - 1) * st;
+  o.p = (q < n - 1) ? st : -(n - 1) * st;
+  return o;
+}
+template <int L, int S>
+__device__ __forceinline__ int nb_off32(const AxisOff32 &oi, const AxisOff32 &oj,
+                                        const AxisOff32 &ok) {
+  constexpr int cx = S * CX(L), cy = S * CY(L), cz = S * CZ(L);
+  int o = 0;
+  if (cx > 0) o += oi.p; else if (cx < 0) o += oi.m;
+  if (cy > 0) o += oj.p; else if (cy < 0) o += oj.m;
+  if (cz > 0) o += ok.p; else if (cz < 0) o += ok.m;
+  return o;
+}
+
 struct Geom {
   int esi, esj, esk, plane;
   long N;

@@ -16,9 +16,11 @@
  *                       state (src/lbstore.c:346-401);
  *   WARM_MOMENTS     -- the post-collide state rebuilt from all 10
  *                       predicted moments (density, momentum, stress).
- * If the residual after the probe interval is not below the cold-start
- * residual recorded for that interval, the populations are reset to rest and
- * the solve restarts cold.
+ * Two safeguards keep a warm solve from costing more than a cold one: a
+ * start policy, which starts cold when the median residual curve of
+ * calibration warm solves reaches the tolerance no sooner than the median
+ * cold curve, and a fallback, which resets the populations to rest when the
+ * residual of a warm solve does not fall fast enough.
  *
  * Everything runs on the GPU: features (jump flooding), forward and
  * backward passes (BF16 implicit-GEMM MFMA convolutions, FP32
@@ -54,7 +56,7 @@ static const int LEVEL_CMAX[NLEVEL] = {16, 32, 64, 64};
 #define WN_SU 1.0
 #define WN_SR 0.125
 #define WN_SS 1.0
-#define WN_MAGIC "SLBWNET3"
+#define WN_MAGIC "SLBWNET4"
 #define WN_CURVE 256     /* checks of the cold reference curve */
 static const int WN_GROUP[WN_TARG] = {0, 0, 0, 1, 2, 2, 2, 2, 2, 2};
 
@@ -94,6 +96,11 @@ struct slb_warmnet {
   double *notes;
   int ncurve, curve_interval;
   double curve[WN_CURVE];
+  /* start policy: median residual curve of calibration warm solves */
+  int nwnote, wnote_interval;
+  double *wnotes;
+  int nwcurve, wcurve_interval;
+  double wcurve[WN_CURVE];
   uint64_t rng;
 };
 
@@ -246,6 +253,7 @@ void slb_warm_free(slb_warmnet *net) {
   }
   free(net->samp);
   free(net->notes);
+  free(net->wnotes);
   free(net);
 }
 
@@ -632,6 +640,7 @@ void slb_warm_set_fallback(slb_warmnet *net, int probe, double probe_res) {
   net->probe = probe;
   net->probe_res = probe_res;
   net->ncurve = 0;
+  net->nwcurve = 0;
 }
 
 void slb_warm_get_fallback(const slb_warmnet *net, int *probe, double *probe_res) {
@@ -648,37 +657,76 @@ static double median(double *v, int n) {
   return v[n / 2];
 }
 
-void slb_warm_note_cold(slb_warmnet *net, const double *res, int n, int interval) {
-  if (n < 1) return;
-  if (net->nnote == 0 || interval != net->note_interval) {
-    net->nnote = 0;
-    net->note_interval = interval;
+/* Append one residual history (res[k] after (k+1)*interval steps, n
+ * entries) to a set of runs and recompute the set's median curve; a set
+ * recorded with another interval starts again. */
+static void add_run(int *count, int *set_interval, double **runs, double *curve,
+                    const double *res, int n, int interval) {
+  if (*count == 0 || interval != *set_interval) {
+    *count = 0;
+    *set_interval = interval;
   }
-  double *nn = realloc(net->notes, sizeof(double) * (size_t)(net->nnote + 1) * WN_CURVE);
+  double *nn = realloc(*runs, sizeof(double) * (size_t)(*count + 1) * WN_CURVE);
   if (!nn) die("out of host memory");
-  net->notes = nn;
-  double *row = net->notes + (size_t)net->nnote * WN_CURVE;
+  *runs = nn;
+  double *row = nn + (size_t)*count * WN_CURVE;
   for (int k = 0; k < WN_CURVE; k++) row[k] = res[k < n ? k : n - 1];
-  net->nnote++;
-  double *t = malloc(sizeof(double) * net->nnote);
+  (*count)++;
+  double *t = malloc(sizeof(double) * *count);
   if (!t) die("out of host memory");
   for (int k = 0; k < WN_CURVE; k++) {
-    for (int i = 0; i < net->nnote; i++) t[i] = net->notes[(size_t)i * WN_CURVE + k];
-    net->curve[k] = median(t, net->nnote);
+    for (int i = 0; i < *count; i++) t[i] = nn[(size_t)i * WN_CURVE + k];
+    curve[k] = median(t, *count);
   }
   free(t);
+}
+
+void slb_warm_note_cold(slb_warmnet *net, const double *res, int n, int interval) {
+  if (n < 1) return;
+  add_run(&net->nnote, &net->note_interval, &net->notes, net->curve, res, n, interval);
   net->ncurve = WN_CURVE;
   net->curve_interval = interval;
   net->probe = 2 * interval;
   net->probe_res = net->curve[1];
 }
 
+void slb_warm_note_warm(slb_warmnet *net, const double *res, int n, int interval) {
+  if (n < 1) return;
+  add_run(&net->nwnote, &net->wnote_interval, &net->wnotes, net->wcurve, res, n, interval);
+  net->nwcurve = WN_CURVE;
+  net->wcurve_interval = interval;
+}
+
+/* Steps a median curve takes to fall below tol (HUGE_VAL if it never does). */
+static double curve_steps(const double *curve, int ncurve, int interval, double tol) {
+  for (int k = 0; k < ncurve; k++)
+    if (curve[k] < tol) return (double)(k + 1) * interval;
+  return HUGE_VAL;
+}
+
+int slb_warm_policy(const slb_warmnet *net, double tol, int interval) {
+  if (net->nwcurve == 0 || net->ncurve == 0 || interval != net->wcurve_interval ||
+      interval != net->curve_interval)
+    return 1;
+  const double w = curve_steps(net->wcurve, net->nwcurve, interval, tol);
+  const double c = curve_steps(net->curve, net->ncurve, interval, tol);
+  return w < c;
+}
+
 int slb_warm_solve(slb_warmnet *net, LBSys lb, double tol, int interval, int maxsteps, double *res,
                    int *used_warm) {
   if (interval < 1) interval = 1;
   double r = -1.0;
-  int steps = 0, warm = 1, done = 0;
-  slb_warm_start(net, lb);
+  int steps = 0, warm = slb_warm_policy(net, tol, interval), done = 0;
+  if (warm) {
+    slb_warm_start(net, lb);
+  } else {
+    /* the calibration says a warm start does not reach tol sooner: start
+     * from rest, as the fallback does */
+    slb_lbpriv *S = PRIV(lb);
+    slbd_zero_pops(S->dev);
+    S->pops_loc = LOC_DEV;
+  }
   slb_residual_reset(lb);
   slb_residual(lb);
   /* cold steps to tol on the reference curve */
@@ -747,6 +795,9 @@ int slb_warm_save(const slb_warmnet *net, const char *path) {
            fwrite(&net->ncurve, sizeof(int), 1, f) == 1 &&
            fwrite(&net->curve_interval, sizeof(int), 1, f) == 1 &&
            fwrite(net->curve, sizeof(double), WN_CURVE, f) == WN_CURVE &&
+           fwrite(&net->nwcurve, sizeof(int), 1, f) == 1 &&
+           fwrite(&net->wcurve_interval, sizeof(int), 1, f) == 1 &&
+           fwrite(net->wcurve, sizeof(double), WN_CURVE, f) == WN_CURVE &&
            fwrite(&net->nparam, sizeof(long), 1, f) == 1 &&
            fwrite(hw, sizeof(float), net->nparam, f) == (size_t)net->nparam;
   free(hw);
@@ -764,13 +815,15 @@ slb_warmnet *slb_warm_load(const char *path) {
   slb_warmnet tmp;
   memset(&tmp, 0, sizeof(tmp));
   layout(&tmp);
-  int ncurve = 0, cint = 0;
-  double curve[WN_CURVE];
+  int ncurve = 0, cint = 0, nwcurve = 0, wcint = 0;
+  double curve[WN_CURVE], wcurve[WN_CURVE];
   int ok = fread(magic, 1, 8, f) == 8 && memcmp(magic, WN_MAGIC, 8) == 0 &&
            fread(hdr, sizeof(hdr), 1, f) == 1 && hdr[0] == NLAYER &&
            fread(&pr, sizeof(double), 1, f) == 1 && fread(&ncurve, sizeof(int), 1, f) == 1 &&
            fread(&cint, sizeof(int), 1, f) == 1 &&
            fread(curve, sizeof(double), WN_CURVE, f) == WN_CURVE &&
+           fread(&nwcurve, sizeof(int), 1, f) == 1 && fread(&wcint, sizeof(int), 1, f) == 1 &&
+           fread(wcurve, sizeof(double), WN_CURVE, f) == WN_CURVE &&
            fread(&np, sizeof(long), 1, f) == 1 && np == tmp.nparam;
   for (int L = 0; ok && L < NLAYER; L++)
     if (hdr[1 + 2 * L] != LCIN[L] || hdr[2 + 2 * L] != LCOUT[L]) ok = 0;
@@ -788,6 +841,9 @@ slb_warmnet *slb_warm_load(const char *path) {
   net->ncurve = ncurve;
   net->curve_interval = cint;
   memcpy(net->curve, curve, sizeof(curve));
+  net->nwcurve = nwcurve;
+  net->wcurve_interval = wcint;
+  memcpy(net->wcurve, wcurve, sizeof(wcurve));
   return net;
 }
 

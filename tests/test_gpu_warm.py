@@ -454,3 +454,55 @@ def test_moment_injection_round_trip():
     A.free()
     B.free()
     assert r < 1e-6 and dv < 1e-6
+
+
+def test_start_policy_weights_file_and_cold_start():
+    """The start policy compares the median warm and cold residual curves at
+    the tolerance; it survives a save/load round trip; set_fallback clears
+    it; and a solve the policy starts cold takes the steps of a cold run."""
+    L = lib()
+    net = L.slb_warm_new(5)
+    D = C.c_double
+    try:
+        assert L.slb_warm_policy(net, 1e-6, 100) == 1  # no curves: warm
+        # cold: res_k = 10^-(k+1)/2 -> below 1e-4 after 900 steps, 1e-6 after 1300
+        cold = [10.0 ** (-(k + 1) / 2.0) for k in range(30)]
+        # warm: res_k = 1e-3 0.6^k -> below 1e-4 after 600 steps, 1e-6 after 1500
+        warm = [1e-3 * 0.6 ** k for k in range(30)]
+        for shift in (1.0, 0.5, 2.0):  # three runs: the median is the middle one
+            L.slb_warm_note_cold(net, (D * 30)(*[x * shift for x in cold]), 30, 100)
+            L.slb_warm_note_warm(net, (D * 30)(*[x * shift for x in warm]), 30, 100)
+        assert L.slb_warm_policy(net, 1e-4, 100) == 1
+        assert L.slb_warm_policy(net, 1e-6, 100) == 0
+        assert L.slb_warm_policy(net, 1e-6, 50) == 1  # curves of another interval
+        with tempfile.TemporaryDirectory() as d:
+            path = os.path.join(d, "net.bin").encode()
+            assert L.slb_warm_save(net, path) == 0
+            net2 = L.slb_warm_load(path)
+            assert net2
+            try:
+                assert L.slb_warm_policy(net2, 1e-4, 100) == 1
+                assert L.slb_warm_policy(net2, 1e-6, 100) == 0
+            finally:
+                L.slb_warm_free(net2)
+        # a solve the policy starts cold: same steps and state as a cold run
+        runs = []
+        for use_net in (True, False):
+            lat = pack_lattice((16, 12, 20), SPHERES, prec=64)
+            if use_net:
+                lat.L.slb_run(lat.h, 7)  # a state that the cold start must discard
+                r, used = D(0.0), C.c_int(-1)
+                steps = L.slb_warm_solve(net, lat.h, 1e-6, 100, 20000, C.byref(r), C.byref(used))
+                assert used.value == 0
+            else:
+                r = D(0.0)
+                steps = lat.L.slb_run_to_residual(lat.h, 1e-6, 100, 20000, C.byref(r))
+            runs.append((steps, r.value, lat.pops().copy()))
+            lat.free()
+        assert runs[0][0] == runs[1][0]
+        assert runs[0][1] == runs[1][1]
+        assert np.array_equal(runs[0][2], runs[1][2])
+        L.slb_warm_set_fallback(net, 0, 0.0)
+        assert L.slb_warm_policy(net, 1e-6, 100) == 1
+    finally:
+        L.slb_warm_free(net)
