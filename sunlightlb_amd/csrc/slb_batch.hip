@@ -89,7 +89,7 @@ k_bstep(const T *__restrict__ src, T *__restrict__ dst,
       Links<1, 15>::run([&](auto LL) {
         constexpr int l = decltype(LL)::value;
         const bool bb = (f >> (l - 1)) & 1u;
-        const int idx = bb ? pidx32(OPP(l), x) : pidx32(l, x + nb_off32<l, -1>(oi, oj, ok));
+        const int idx = bb ? pidx32(OPP(l), x) : pidx32(l, x + nb_off<l, -1>(oi, oj, ok));
         v[l] = s[idx];
       });
       if ((P.moving || FORCE) && (f & FLAG_LINKS)) {
@@ -97,7 +97,7 @@ k_bstep(const T *__restrict__ src, T *__restrict__ dst,
         Links<1, 15>::run([&](auto LL) {
           constexpr int l = decltype(LL)::value;
           if ((f >> (l - 1)) & 1u) {
-            const int xs = x + nb_off32<l, -1>(oi, oj, ok);
+            const int xs = x + nb_off<l, -1>(oi, oj, ok);
             T cu = T(0);
             if (P.moving) {
               /* wall velocity U = (rho0/12)(v + w x r) of the solid end of

@@ -41,7 +41,9 @@
 /* ---------------------------------------------------------------------- */
 /* Fused step kernel.                                                      */
 
-template <typename T, bool MOVING, bool FORCE, int BF>
+/* I: index type, int when the lattice holds fewer than 2^31 populations
+ * (half the address arithmetic), long otherwise. */
+template <typename T, bool MOVING, bool FORCE, int BF, typename I>
 __global__ void __launch_bounds__(512)
 k_step(const T *__restrict__ src, T *__restrict__ dst,
        const uint16_t *__restrict__ flags,
@@ -57,28 +59,27 @@ k_step(const T *__restrict__ src, T *__restrict__ dst,
   const int i = blk_y;
   const int j = t / G.esk;
   const int k = t - j * G.esk;
-  const long N = G.N;
-  const long x = (long)i * G.plane + t;
-  const AxisOff oi = axis_off(i, G.esi, G.plane);
-  const AxisOff oj = axis_off(j, G.esj, G.esk);
-  const AxisOff ok = axis_off(k, G.esk, 1);
+  const I x = (I)i * (I)G.plane + (I)t;
+  const AxisOffT<I> oi = axis_off_t<I>(i, G.esi, (I)G.plane);
+  const AxisOffT<I> oj = axis_off_t<I>(j, G.esj, (I)G.esk);
+  const AxisOffT<I> ok = axis_off_t<I>(k, G.esk, (I)1);
   const unsigned fl = flags[x];
   const bool solid = (fl & FLAG_SOLID) != 0;
   T v[15];
   if (G.nt_load) {
-    v[0] = __builtin_nontemporal_load(&src[pidx(0, x)]);
+    v[0] = __builtin_nontemporal_load(&src[pidx_t<I>(0, x)]);
     Links<1, 15>::run([&](auto LL) {
       constexpr int l = decltype(LL)::value;
       const bool bb = (fl >> (l - 1)) & 1u;
-      const long idx = bb ? pidx(OPP(l), x) : pidx(l, x + nb_off<l, -1>(oi, oj, ok));
+      const I idx = bb ? pidx_t<I>(OPP(l), x) : pidx_t<I>(l, x + nb_off<l, -1>(oi, oj, ok));
       v[l] = __builtin_nontemporal_load(&src[idx]);
     });
   } else {
-    v[0] = src[pidx(0, x)];
+    v[0] = src[pidx_t<I>(0, x)];
     Links<1, 15>::run([&](auto LL) {
       constexpr int l = decltype(LL)::value;
       const bool bb = (fl >> (l - 1)) & 1u;
-      const long idx = bb ? pidx(OPP(l), x) : pidx(l, x + nb_off<l, -1>(oi, oj, ok));
+      const I idx = bb ? pidx_t<I>(OPP(l), x) : pidx_t<I>(l, x + nb_off<l, -1>(oi, oj, ok));
       v[l] = src[idx];
     });
   }
@@ -87,17 +88,17 @@ k_step(const T *__restrict__ src, T *__restrict__ dst,
     Links<1, 15>::run([&](auto LL) {
       constexpr int l = decltype(LL)::value;
       if ((fl >> (l - 1)) & 1u) {
-        const long xs = x + nb_off<l, -1>(oi, oj, ok);
+        const I xs = x + nb_off<l, -1>(oi, oj, ok);
         double cu = 0.0;
         if (MOVING) {
-          const long s = solid ? x : xs;
+          const I s = solid ? x : xs;
           cu = (double)WL(l) * ((CX(l) ? CX(l) * ux[s] : 0.0) + (CY(l) ? CY(l) * uy[s] : 0.0) +
                                 (CZ(l) ? CZ(l) * uz[s] : 0.0));
         }
         if (FORCE && solid) {
           /* link m = lbar of solid x: a = a_m(x), b = a_l(x - c_l) */
           const double a = (double)v[l];
-          const double b = (double)src[pidx(l, xs)];
+          const double b = (double)src[pidx_t<I>(l, xs)];
           const double g = -2.0 * (a - b + cu);
           if (CX(l)) Fx += CX(l) * g;
           if (CY(l)) Fy += CY(l) * g;
@@ -120,10 +121,10 @@ k_step(const T *__restrict__ src, T *__restrict__ dst,
   }
   if (G.nt_store) {
 #pragma unroll
-    for (int l = 0; l < 15; l++) __builtin_nontemporal_store(v[l], &dst[pidx(l, x)]);
+    for (int l = 0; l < 15; l++) __builtin_nontemporal_store(v[l], &dst[pidx_t<I>(l, x)]);
   } else {
 #pragma unroll
-    for (int l = 0; l < 15; l++) dst[pidx(l, x)] = v[l];
+    for (int l = 0; l < 15; l++) dst[pidx_t<I>(l, x)] = v[l];
   }
 }
 
@@ -1075,9 +1076,14 @@ static void launch_step_t(slbd_lattice *L, const PhysArgs &A) {
   const T *src = (const T *)L->pops[L->cur];
   T *dst = (T *)L->pops[L->cur ^ 1];
   const dim3 grid((unsigned)((L->G.plane + L->G.step_tpb - 1) / L->G.step_tpb), (unsigned)L->esi);
-  hipLaunchKernelGGL((k_step<T, MOVING, FORCE, BF>), grid, dim3(L->G.step_tpb), 0, L->stream,
-                     src, dst, L->flags, L->ux, L->uy, L->uz, (const T *)L->bx,
-                     (const T *)L->by, (const T *)L->bz, L->fx, L->fy, L->fz, L->G, A);
+  if (15 * pop_nodes(L->G.N) <= 0x7fffffffL)
+    hipLaunchKernelGGL((k_step<T, MOVING, FORCE, BF, int>), grid, dim3(L->G.step_tpb), 0,
+                       L->stream, src, dst, L->flags, L->ux, L->uy, L->uz, (const T *)L->bx,
+                       (const T *)L->by, (const T *)L->bz, L->fx, L->fy, L->fz, L->G, A);
+  else
+    hipLaunchKernelGGL((k_step<T, MOVING, FORCE, BF, long>), grid, dim3(L->G.step_tpb), 0,
+                       L->stream, src, dst, L->flags, L->ux, L->uy, L->uz, (const T *)L->bx,
+                       (const T *)L->by, (const T *)L->bz, L->fx, L->fy, L->fz, L->G, A);
   LAUNCHCHECK();
   L->cur ^= 1;
 }

@@ -58,9 +58,11 @@ __host__ __device__ constexpr int OPP(int l) {
 #define POP_SHIFT 10
 #endif
 #define POP_TILE (1L << POP_SHIFT)
-__host__ __device__ __forceinline__ long pidx(int l, long x) {
-  return ((x >> POP_SHIFT) * 15L + l) * POP_TILE + (x & (POP_TILE - 1));
+template <typename I>
+__host__ __device__ __forceinline__ I pidx_t(int l, I x) {
+  return ((x >> POP_SHIFT) * I(15) + I(l)) * I(POP_TILE) + (x & I(POP_TILE - 1));
 }
+__host__ __device__ __forceinline__ long pidx(int l, long x) { return pidx_t<long>(l, x); }
 __host__ __device__ __forceinline__ long pop_nodes(long N) {
   return (N + POP_TILE - 1) & ~(POP_TILE - 1);
 }
@@ -72,49 +74,39 @@ __host__ __device__ __forceinline__ long pop_nodes(long N) {
  * length n and stride st: off[0] for component -1 (source q+1), off[2] for
  * component +1 (source q-1); used with sign = -1 for pull (x - c) or +1
  * for push (x + c). */
-struct AxisOff {
-  long m, p;  /* offset of q-1 and q+1 with periodic wrap */
+template <typename I>
+struct AxisOffT {
+  I m, p;  /* offset of q-1 and q+1 with periodic wrap */
 };
-__device__ __forceinline__ AxisOff axis_off(int q, int n, long st) {
-  AxisOff o;
-  o.m = (q > 0) ? -st : (long)(n - 1) * st;
-  o.p = (q < n - 1) ? st : -(long)(n - 1) * st;
+typedef AxisOffT<long> AxisOff;
+template <typename I>
+__device__ __forceinline__ AxisOffT<I> axis_off_t(int q, int n, I st) {
+  AxisOffT<I> o;
+  o.m = (q > 0) ? -st : I(n - 1) * st;
+  o.p = (q < n - 1) ? st : -I(n - 1) * st;
   return o;
 }
+__device__ __forceinline__ AxisOff axis_off(int q, int n, long st) {
+  return axis_off_t<long>(q, n, st);
+}
 /* offset of x + s*c_l (s = +1 push, -1 pull) */
-template <int L, int S>
-__device__ __forceinline__ long nb_off(const AxisOff &oi, const AxisOff &oj, const AxisOff &ok) {
+template <int L, int S, typename I>
+__device__ __forceinline__ I nb_off(const AxisOffT<I> &oi, const AxisOffT<I> &oj,
+                                    const AxisOffT<I> &ok) {
   constexpr int cx = S * CX(L), cy = S * CY(L), cz = S * CZ(L);
-  long o = 0;
+  I o = 0;
   if (cx > 0) o += oi.p; else if (cx < 0) o += oi.m;
   if (cy > 0) o += oj.p; else if (cy < 0) o += oj.m;
   if (cz > 0) o += ok.p; else if (cz < 0) o += ok.m;
   return o;
 }
 
-/* 32-bit forms of the above, for lattices of fewer than 2^31 / 15
- * populations (the batched boxes): half the address registers. */
-__host__ __device__ __forceinline__ int pidx32(int l, int x) {
-  return ((x >> POP_SHIFT) * 15 + l) * (int)POP_TILE + (x & (int)(POP_TILE - 1));
-}
-struct AxisOff32 {
-  int m, p;
-};
+/* 32-bit forms of the above, for lattices of fewer than 2^31 populations:
+ * half the address registers and arithmetic. */
+__host__ __device__ __forceinline__ int pidx32(int l, int x) { return pidx_t<int>(l, x); }
+typedef AxisOffT<int> AxisOff32;
 __device__ __forceinline__ AxisOff32 axis_off32(int q, int n, int st) {
-  AxisOff32 o;
-  o.m = (q > 0) ? -st : (n - 1) * st;
-  o.p = (q < n - 1) ? st : -(n - 1) * st;
-  return o;
-}
-template <int L, int S>
-__device__ __forceinline__ int nb_off32(const AxisOff32 &oi, const AxisOff32 &oj,
-                                        const AxisOff32 &ok) {
-  constexpr int cx = S * CX(L), cy = S * CY(L), cz = S * CZ(L);
-  int o = 0;
-  if (cx > 0) o += oi.p; else if (cx < 0) o += oi.m;
-  if (cy > 0) o += oj.p; else if (cy < 0) o += oj.m;
-  if (cz > 0) o += ok.p; else if (cz < 0) o += ok.m;
-  return o;
+  return axis_off_t<int>(q, n, st);
 }
 
 struct Geom {
