@@ -554,7 +554,7 @@ struct MPArgs {
  *   pi_l(y) = a_l(y)/rho(y) + (w_l/72)(rho0/rho(y) - alpha) (+ alpha, l = 0)
  * then the update rule of src/mpkernel.c:190-197 (fluid replace, adsorbing
  * site accumulate, other solid clear; partial adsorption keeps pads*new). */
-template <typename T, int NMOM, bool MOVING>
+template <typename T, int NMOM, bool MOVING, typename I>
 __global__ void __launch_bounds__(256)
 k_mpstep(const T *__restrict__ a, const T *__restrict__ qin, T *__restrict__ qout,
          const uint32_t *__restrict__ mpf, const int32_t *__restrict__ mask,
@@ -563,13 +563,20 @@ k_mpstep(const T *__restrict__ a, const T *__restrict__ qin, T *__restrict__ qou
   constexpr int W = NMOM > 1 ? 4 : 2;
   typedef T V __attribute__((ext_vector_type(W)));
   const V *__restrict__ Q = (const V *)qin;
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  /* mp_xcd 1: each XCD walks a contiguous range of planes, so the records of
+   * planes i - 1 and i + 1 that a plane gathers were loaded into the same
+   * XCD's L2 by the neighbouring planes */
+  Geom Gb = G;
+  Gb.xcd_map = G.mp_xcd;
+  int blk_x, blk_y;
+  step_block(Gb, blk_x, blk_y);
+  const int t = blk_x * blockDim.x + threadIdx.x;
   if (t >= G.plane) return;
-  const int i = blockIdx.y, j = t / G.esk, k = t - j * G.esk;
-  const long x = (long)i * G.plane + t;
-  const AxisOff oi = axis_off(i, G.esi, G.plane);
-  const AxisOff oj = axis_off(j, G.esj, G.esk);
-  const AxisOff ok = axis_off(k, G.esk, 1);
+  const int i = blk_y, j = t / G.esk, k = t - j * G.esk;
+  const I x = (I)i * (I)G.plane + (I)t;
+  const AxisOffT<I> oi = axis_off_t<I>(i, G.esi, (I)G.plane);
+  const AxisOffT<I> oj = axis_off_t<I>(j, G.esj, (I)G.esk);
+  const AxisOffT<I> ok = axis_off_t<I>(k, G.esk, (I)1);
   const uint32_t f = mpf[x];
   const T r0 = T(A.rho0), al = T(A.alpha);
   const V qx = Q[x];
@@ -577,7 +584,7 @@ k_mpstep(const T *__restrict__ a, const T *__restrict__ qin, T *__restrict__ qou
   T np = T(0), n1 = T(0), n2 = T(0);
   /* rest link: x itself */
   if (f & 1u) {
-    const T pr = a[pidx(0, x)] * rix + T(16.0 / 72.0) * (r0 * rix - al) + al;
+    const T pr = a[pidx_t<I>(0, x)] * rix + T(16.0 / 72.0) * (r0 * rix - al) + al;
     np += pr * qx[0];
     if (NMOM > 1) { n1 += pr * qx[1]; n2 += pr * qx[2]; }
   }
@@ -589,9 +596,9 @@ k_mpstep(const T *__restrict__ a, const T *__restrict__ qin, T *__restrict__ qou
   T av[15];
   Links<1, 15>::run([&](auto LL) {
     constexpr int l = decltype(LL)::value;
-    const long y = x + nb_off<l, -1>(oi, oj, ok);
+    const I y = x + nb_off<l, -1>(oi, oj, ok);
     qv[l] = Q[y];
-    av[l] = a[pidx(l, y)];
+    av[l] = a[pidx_t<I>(l, y)];
   });
   Links<1, 15>::run([&](auto LL) {
     constexpr int l = decltype(LL)::value;
@@ -612,14 +619,14 @@ k_mpstep(const T *__restrict__ a, const T *__restrict__ qin, T *__restrict__ qou
     const T keep = T(1.0 - A.pads);
     Links<1, 15>::run([&](auto LL) {
       constexpr int l = decltype(LL)::value;
-      av[l] = a[pidx(OPP(l), x)];
+      av[l] = a[pidx_t<I>(OPP(l), x)];
     });
     Links<1, 15>::run([&](auto LL) {
       constexpr int l = decltype(LL)::value;
       if (f & (1u << (16 + l))) {
         T pr = av[l] * rix + T(WL(l) / 72.0) * (r0 * rix - al);
         if (MOVING) {
-          const long s = x + nb_off<l, -1>(oi, oj, ok);
+          const I s = x + nb_off<l, -1>(oi, oj, ok);
           const int si = (i - CX(l) + G.esi) % G.esi;
           const int sj = (j - CY(l) + G.esj) % G.esj;
           const int sk = (k - CZ(l) + G.esk) % G.esk;
@@ -819,6 +826,12 @@ extern "C" slbd_lattice *slbd_create(int esi, int esj, int esk, int prec) {
     if (b == 256 || b == 512) L->G.step_tpb = b;
   }
   L->G.xcd_map = step_order(L->G);
+  {
+    /* MP kernel: 256-thread blocks over (row block, plane) */
+    const long nb = (long)((L->G.plane + 255) / 256) * esi;
+    const int want = getenv("SLB_MP_XCD_MAP") ? atoi(getenv("SLB_MP_XCD_MAP")) : 1;
+    L->G.mp_xcd = (want == 1 && nb % 8 == 0) ? 1 : 0;
+  }
   HIPCHECK(hipStreamCreateWithFlags(&L->stream, hipStreamNonBlocking));
   HIPCHECK(hipEventCreate(&L->ev0));
   HIPCHECK(hipEventCreate(&L->ev1));
@@ -1471,9 +1484,15 @@ extern "C" void slbd_mp_zero(slbd_lattice *L, slbd_mp *M, int field) {
 template <typename T, int NMOM, bool MOVING>
 static void launch_mp_t(slbd_lattice *L, slbd_mp *M, const MPArgs &A) {
   const int c = M->cur;
-  hipLaunchKernelGGL((k_mpstep<T, NMOM, MOVING>), plane_grid(L), dim3(256), 0, L->stream,
-                     (const T *)L->pops[L->cur], (const T *)M->q[c], (T *)M->q[c ^ 1], L->mpf,
-                     L->mask, L->objidx, L->tab, L->G, A);
+  /* 32-bit indices pay off (fewer VGPRs) only without the wall term */
+  if (!MOVING && 15 * pop_nodes(L->G.N) <= 0x7fffffffL)
+    hipLaunchKernelGGL((k_mpstep<T, NMOM, MOVING, int>), plane_grid(L), dim3(256), 0, L->stream,
+                       (const T *)L->pops[L->cur], (const T *)M->q[c], (T *)M->q[c ^ 1], L->mpf,
+                       L->mask, L->objidx, L->tab, L->G, A);
+  else
+    hipLaunchKernelGGL((k_mpstep<T, NMOM, MOVING, long>), plane_grid(L), dim3(256), 0,
+                       L->stream, (const T *)L->pops[L->cur], (const T *)M->q[c],
+                       (T *)M->q[c ^ 1], L->mpf, L->mask, L->objidx, L->tab, L->G, A);
   LAUNCHCHECK();
   M->cur ^= 1;
 }

@@ -117,6 +117,7 @@ struct Geom {
   int nt_store;  /* step kernel: non-temporal population stores */
   int nt_load;   /* step kernel: non-temporal population loads */
   int step_tpb;  /* step kernel: threads per block (256 or 512) */
+  int mp_xcd;    /* moment-propagation kernel block order (as xcd_map: 0 or 1) */
 };
 
 /* Block coordinates of the step kernel (blockIdx.x: block of rows within a
