@@ -104,8 +104,9 @@ void wrpars(LBSys lb) {
   else printf(" Relaxation on fluid and solid nodes\n");
   if (lb->objects_moving) printf(" Objects allowed to move\n");
   else printf(" All objects are stationary\n");
-  printf(" GPU %s, %d-bit populations, %.2f MByte on device\n", slbd_arch(),
-         PRIV(lb)->prec, (double)slb_device_bytes(lb) / (1024 * 1024));
+  /* device details go to stderr: stdout keeps the reference's lines */
+  fprintf(stderr, " GPU %s, %d-bit populations, %.2f MByte on device\n", slbd_arch(),
+          PRIV(lb)->prec, (double)slb_device_bytes(lb) / (1024 * 1024));
 #ifdef SIGNAL
   printf(" Signalling implemented, process id = %i\n", (int)getpid());
 #else

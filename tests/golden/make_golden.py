@@ -291,6 +291,139 @@ def case_vacf(R):
     print("wrote vacf")
 
 
+def case_sphere_c(R):
+    """The reference example driver examples/sphere.c (16^3, one sphere, Stokes,
+    skip_solid, tau 0.8, 1000 steps, measure) replayed call for call on the
+    reference library.  sphere.c sets the uniform body force through
+    mylb->dpdx, which the current reference header lacks; here it is the
+    equivalent addbf(0.01, 0, 0) on the fluid nodes after the insertion.
+    Stores the five files the driver writes and its stdout."""
+    import gzip
+    import tempfile
+    L = R.L
+    vp, sp = C.c_void_p, C.c_char_p
+    for n, a in {"about": [], "wrlinks": [vp], "wrmap": [vp, sp], "lbsavemap": [vp, sp, sp],
+                 "wrpars": [vp], "wrmon": [vp, sp], "wrstats": [vp], "wrmeas": [vp],
+                 "wrobjs": [vp], "lbsavefrc": [vp, sp, sp]}.items():
+        getattr(L, n).argtypes = a
+        getattr(L, n).restype = None
+    libc = C.CDLL(None)
+    cwd = os.getcwd()
+    with tempfile.TemporaryDirectory() as d:
+        os.chdir(d)
+        sys.stdout.flush()
+        saved = os.dup(1)
+        fd = os.open("stdout.txt", os.O_WRONLY | os.O_CREAT | os.O_TRUNC)
+        os.dup2(fd, 1)
+        try:
+            L.about()
+            h = L.newlb(16, 16, 16, b"sphere_c")
+            L.set_verbosity(0x04)  # OBJECT_DETAILS
+            st = h.contents
+            st.stokes_flow, st.objects_moving, st.skip_solid, st.tau = 1, 0, 1, 0.80
+            L.wrlinks(h)
+            L.insertsph(h, 3.7, 7.5, 7.5, 7.5)
+            L.addbf(h, 0.01, 0.0, 0.0)
+            L.wrmap(h, b"sphere_c.map")
+            L.lbsavemap(h, b"sphere_c.dat", b"list")
+            L.wrpars(h)
+            L.wrmon(h, b"initialised")
+            libc.printf(b"\nEquilibrating for %i steps\n", C.c_int(1000))
+            for _ in range(1000):
+                L.lbstep(h)
+            L.wrstats(h)
+            L.measure(h)
+            L.wrmeas(h)
+            L.wrobjs(h)
+            L.lbsavefrc(h, b"sphere_c.frc", b"list")
+            L.lbsavevel(h, b"sphere_c.vel", b"ascii")
+            L.wrmon(h, b"finished")
+            libc.fflush(None)
+        finally:
+            os.dup2(saved, 1)
+            os.close(fd)
+            os.close(saved)
+        out = {"stdout": open("stdout.txt").read(), "files": {}}
+        for f in ("sphere_c.map", "sphere_c.dat", "sphere_c.frc", "sphere_c.vel", "sphere_c.mon"):
+            out["files"][f] = open(f).read()
+        os.chdir(cwd)
+    with gzip.open(os.path.join(OUT, "sphere_c.json.gz"), "wt") as f:
+        json.dump(out, f)
+    print("wrote sphere_c")
+
+
+IMG = dict(dims=(12, 10, 9), sphere=(3.0, 5.5, 4.5, 4.0), vel=(0.02, -0.005, 0.01), tau=0.8,
+           steps=30, mp_steps=20, slice=4,
+           vmap=(-0.01, 0.03, (10, 50, 50), (250, 200, 200)),
+           pmap=(0.0, 2.0, (0, 0, 0), (200, 100, 50)),
+           back=(255, 255, 255), obj=(0, 0, 0))
+
+
+def case_images(R):
+    """src/mpimage.c on a slice normal to z of a lattice the size of the image
+    (where its pixel placement is right), a sphere moving through fluid at
+    rest with no body force (its velocity correction then adds nothing) and
+    green_lo == blue_lo (its blue channel starts from green_lo): the solid
+    map, the three velocity components and the speed, and an MP field added
+    to and subtracted from the map."""
+    L = R.L
+    vp, sp, I = C.c_void_p, C.c_char_p, C.c_int
+    D = C.c_double
+    for n, (r, a) in {"newlbcolour": (vp, [I, I, I]), "newlbcolourmap": (vp, [D, D, vp, vp]),
+                      "newlbimage": (vp, [I, I, vp, vp]), "lbimageclear": (None, [vp]),
+                      "lbimagemap": (None, [vp, vp, sp, I]),
+                      "lbimagevel": (None, [vp, vp, vp, sp, sp, I]),
+                      "mpimageset": (None, [vp, vp, vp, sp, sp, I])}.items():
+        getattr(L, n).restype = r
+        getattr(L, n).argtypes = a
+    c = IMG
+    h = R.new(*c["dims"])
+    st = h.contents
+    st.tau, st.stokes_flow, st.objects_moving = c["tau"], 1, 1
+    o = L.insertsph(h, *c["sphere"])
+    o.contents.velx, o.contents.vely, o.contents.velz = c["vel"]
+    L.vspread(h)
+    for _ in range(c["steps"]):
+        L.lbstep(h)
+    esi, esj, esk = c["dims"]
+    back, obj = L.newlbcolour(*c["back"]), L.newlbcolour(*c["obj"])
+    im = L.newlbimage(esi, esj, back, obj)
+    im_c = C.cast(im, C.POINTER(C.c_int * 2)).contents
+    area = esi * esj
+
+    def pixels():
+        ptrs = C.cast(im + 8, C.POINTER(C.c_void_p * 3)).contents
+        return np.stack([np.ctypeslib.as_array(C.cast(ptrs[q], C.POINTER(C.c_int)), shape=(area,))
+                         .copy() for q in range(3)])
+    assert tuple(im_c) == (esi, esj)
+    out = {"meta": json.dumps(c)}
+    L.lbimagemap(im, h, b"z", c["slice"])
+    out["map"] = pixels()
+    lo, hi = L.newlbcolour(*c["vmap"][2]), L.newlbcolour(*c["vmap"][3])
+    vmap = L.newlbcolourmap(c["vmap"][0], c["vmap"][1], lo, hi)
+    for w in ("x", "y", "z", "m"):
+        L.lbimagevel(im, vmap, h, w.encode(), b"z", c["slice"])
+        out[f"vel_{w}"] = pixels()
+    mp = L.newmp(h, b"null")
+    L.mprhoinit(mp)
+    L.mpinit(mp)
+    L.mpall(mp, 1.0)
+    for _ in range(c["mp_steps"]):
+        L.mpstep(mp)
+    plo, phi = L.newlbcolour(*c["pmap"][2]), L.newlbcolour(*c["pmap"][3])
+    pmap = L.newlbcolourmap(c["pmap"][0], c["pmap"][1], plo, phi)
+    L.lbimagemap(im, h, b"z", c["slice"])
+    L.mpimageset(im, mp, pmap, b"a", b"z", c["slice"])
+    out["mp_add"] = pixels()
+    L.mpimageset(im, mp, pmap, b"s", b"z", c["slice"])
+    out["mp_sub"] = pixels()
+    N = st.nnodes
+    out["a"] = R.arr(st.a, 15 * N).astype(np.float64).reshape(N, 15).copy()
+    out["p"] = R.arr(mp.contents.p, N).astype(np.float64).copy()
+    np.savez_compressed(os.path.join(OUT, "images.npz"), **out)
+    print("wrote images")
+
+
 def main():
     R = RefLib(64)
     R32 = RefLib(32)
@@ -305,6 +438,8 @@ def main():
     case_loadvel(R)
     case_config1(R, R32)
     case_vacf(R)
+    case_sphere_c(R)
+    case_images(R)
 
 
 if __name__ == "__main__":
