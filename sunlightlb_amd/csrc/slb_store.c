@@ -218,7 +218,10 @@ void lbloadvel(LBSys lb, char *loadfile, char *type) {
 }
 
 /* lbsavestr: stress averaged over the post-propagate and post-collide states
- * of one step, plus a void-space flag (src/lbstore.c:412-515). */
+ * of one step, plus a void-space flag (src/lbstore.c:412-515).  The binary
+ * records hold the stresses converted to LBStoreReal; the reference writes
+ * the first sizeof(LBStoreReal) bytes of each LBReal instead (:88-93), which
+ * with float storage of double stresses is not the value. */
 void lbsavestr(LBSys lb, char *savefile, char *type) {
   long N = lb->nnodes;
   LBReal *acc = (LBReal *)malloc(sizeof(LBReal) * 6 * (size_t)N);

@@ -424,6 +424,80 @@ def case_images(R):
     print("wrote images")
 
 
+DIAG = dict(dims=(11, 9, 10), sphere=(2.8, 5.0, 4.0, 5.5), bf=(2e-4, 0.0, 5e-5), tau=0.9,
+            steps=25, mp_sphere=(3.5, 4.0, 5.0, 4.0, 1.0), mp_steps=12, iso=0.05,
+            p2byte=(0.0, 0.8), adt_mult=10)
+
+
+def case_diagnostics(R):
+    """The diagnostic writers on a Navier-Stokes lattice with a sphere and a
+    body force: lbsavestr (ascii and binary; each call advances one step),
+    and on an MP field started as a sphere of tracer with adsorbing solid
+    sites -- mpconwr (ascii with a step-numbered name, binary, byte-converted),
+    mpspread along x, y, z, mpcheck's totals, mpmean, adtotal written to a
+    file twice, the cumulative isosurface of mpiso/mpisowr, and mpsave."""
+    import tempfile
+    L = R.L
+    vp, sp, I = C.c_void_p, C.c_char_p, C.c_int
+    D = C.c_double
+    for n, (r, a) in {"lbsavestr": (None, [vp, sp, sp]), "mpconwr": (None, [vp, sp, sp]),
+                      "mpspread": (D, [vp, sp]), "mpmean": (D, [vp]),
+                      "mpsph": (None, [vp, D, D, D, D, D]), "newmpiso": (vp, [vp, D]),
+                      "mpiso": (None, [vp]), "mpisowr": (None, [vp, sp, sp]),
+                      "mpsave": (None, [vp, sp, sp])}.items():
+        getattr(L, n).restype = r
+        getattr(L, n).argtypes = a
+    c = DIAG
+    h = R.new(*c["dims"])
+    st = h.contents
+    st.tau, st.stokes_flow = c["tau"], 0
+    L.insertsph(h, *c["sphere"])
+    L.zerobf(h)
+    L.addbf(h, *c["bf"])
+    for _ in range(c["steps"]):
+        L.lbstep(h)
+    out = {"meta": json.dumps(c)}
+    cwd = os.getcwd()
+    with tempfile.TemporaryDirectory() as d:
+        os.chdir(d)
+        try:
+            L.lbsavestr(h, b"str.txt", b"ascii")
+            L.lbsavestr(h, b"str.bin", b"binary")
+            N = st.nnodes
+            out["a"] = R.arr(st.a, 15 * N).astype(np.float64).reshape(N, 15).copy()
+            mp = L.newmp(h, b"mpd")
+            mp.contents.alpha = 0.0  # newmp leaves alpha unset (src/mpmisc.c:38-59)
+            L.mprhoinit(mp)
+            L.mpinit(mp)
+            L.mpsph(C.cast(mp, vp), *c["mp_sphere"])
+            L.adall(mp)
+            iso = L.newmpiso(C.cast(mp, vp), c["iso"])
+            for _ in range(c["mp_steps"]):
+                L.mpstep(mp)
+                L.mpiso(iso)
+            mp.contents.p2bytemin, mp.contents.p2bytemax = c["p2byte"]
+            L.mpconwr(C.cast(mp, vp), b"con_%i.txt", b"ua")
+            L.mpconwr(C.cast(mp, vp), b"con.bin", b"nwb")
+            L.mpconwr(C.cast(mp, vp), b"con.byte", b"nwc")
+            out["spread"] = np.array([L.mpspread(C.cast(mp, vp), x) for x in (b"x", b"y", b"z")])
+            out["mean"] = np.array(L.mpmean(C.cast(mp, vp)))
+            out["adt"] = np.array([L.adtotal(mp, b"adt.txt", c["adt_mult"]) for _ in range(2)])
+            L.mpisowr(iso, b"iso.txt", b"ascii")
+            L.mpsave(C.cast(mp, vp), b"mp.txt", b"ascii")
+            out["p"] = R.arr(mp.contents.p, N).astype(np.float64).copy()
+            step = mp.contents.step
+            out["con_name"] = np.array(f"con_{step}.txt")
+            for k, f in (("str_txt", "str.txt"), ("con_txt", f"con_{step}.txt"), ("adt_txt", "adt.txt"),
+                         ("iso_txt", "iso.txt"), ("mp_txt", "mp.txt")):
+                out[k] = np.array(open(f).read())
+            for k, f in (("str_bin", "str.bin"), ("con_bin", "con.bin"), ("con_byte", "con.byte")):
+                out[k] = np.frombuffer(open(f, "rb").read(), dtype=np.uint8).copy()
+        finally:
+            os.chdir(cwd)
+    np.savez_compressed(os.path.join(OUT, "diagnostics.npz"), **out)
+    print("wrote diagnostics")
+
+
 def main():
     R = RefLib(64)
     R32 = RefLib(32)
@@ -440,6 +514,7 @@ def main():
     case_vacf(R)
     case_sphere_c(R)
     case_images(R)
+    case_diagnostics(R)
 
 
 if __name__ == "__main__":
