@@ -82,6 +82,85 @@ def cpu_baseline(L, h, n, force, steps):
                        f"{dt:.1f} s")
 
 
+def ref_from_map(R, L, h, n):
+    """A reference lattice holding the solid map of lattice h (n^3)."""
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "map.dat")
+        L.lbsavemap(h, path.encode(), b"char")
+        rh = R.L.newlb(n, n, n, b"null")
+        R.L.insertsubs(rh, path.encode(), 0, n - 1, 0, n - 1, 0, n - 1)
+    return rh
+
+
+def cpu_mp_baseline(L, h, n, steps):
+    """Time the reference's mpstep (FP32 build, one core) on the same solid
+    map: uniform tracer, reflecting walls (the reference has no partial
+    adsorption; the cost per node is the same), frozen flow at rest."""
+    from oracle import RefLib
+    try:
+        R = RefLib(32)
+    except ImportError as e:
+        log(f"cpu_mp_baseline: reference library unavailable ({e})")
+        return None
+    rh = ref_from_map(R, L, h, n)
+    rh.contents.stokes_flow = 1
+    R.L.lbstep(rh)
+    mp = R.L.newmp(rh, b"null")
+    mp.contents.alpha = 0.0
+    R.L.mprhoinit(mp)
+    R.L.mpinit(mp)
+    R.L.mpall(mp, 1.0)
+    R.L.mpstep(mp)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        R.L.mpstep(mp)
+    dt = time.perf_counter() - t0
+    return dict(value=round(n ** 3 * steps / dt / 1e6, 3), unit="MLUPS", cores=1,
+                kind="reference",
+                sample=f"reference mpstep (-D_SINGLE, serial) on the same {n}^3 map, {steps} "
+                       f"steps after one untimed step, {dt:.1f} s")
+
+
+def cpu_batch_baseline(boxes, steps, u0, tau, nbox=16):
+    """Time the reference's lbstep (FP32 build, one core) on the first boxes of
+    the batch, sphere 0 and its images moving along x (one right-hand side
+    per box; the other five cost the same).  The boxes are independent, so
+    a host with C cores runs C of them at once."""
+    from oracle import RefLib
+    try:
+        R = RefLib(32)
+    except ImportError as e:
+        log(f"cpu_batch_baseline: reference library unavailable ({e})")
+        return None
+    nodes, dt = 0, 0.0
+    for (Lb, sph) in boxes[:nbox]:
+        rh = R.L.newlb(Lb, Lb, Lb, b"null")
+        st = rh.contents
+        st.tau, st.stokes_flow, st.objects_moving = tau, 1, 1
+        for q in range(len(sph) - 1, -1, -1):
+            Rq, c = sph[q][0], sph[q][1:]
+            for a in (-1, 0, 1):
+                for b_ in (-1, 0, 1):
+                    for e in (-1, 0, 1):
+                        o = (c[0] + a * Lb, c[1] + b_ * Lb, c[2] + e * Lb)
+                        if all(-Rq < o[d] < Lb + Rq for d in range(3)):
+                            p = R.L.insertsph(rh, Rq, *o)
+                            if p and q == 0:
+                                p.contents.velx = u0
+        R.L.vspread(rh)
+        R.L.lbstep(rh)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            R.L.lbstep(rh)
+        dt += time.perf_counter() - t0
+        nodes += Lb ** 3
+    return dict(value=round(nodes * steps / dt / 1e6, 3), unit="MLUPS", cores=1,
+                kind="reference",
+                sample=f"reference lbstep with moving walls (-D_SINGLE, serial) on the first "
+                       f"{min(nbox, len(boxes))} boxes of the batch, one right-hand side each, "
+                       f"{steps} steps per box, {dt:.1f} s")
+
+
 def traffic_bytes(n, precision):
     """HBM bytes per k_step launch measured by rocprofv3 --pmc FETCH_SIZE and
     WRITE_SIZE on this workload, each scaled by the factor the same counter
@@ -161,7 +240,8 @@ def mp_workload(L, args, n=128, R=6.0, phi=0.25, peclet=10.0, alpha=0.0, pads=0.
     var = s1[4] / ptot - mean * mean
     N = n ** 3
     mlups = N * args.mp_steps / (ms * 1e-3) / 1e6
-    bpl = (15 + 1 + 3 * 2) * (4 if args.precision == 32 else 8) + 4
+    # 15 populations + one 4-value node record {P, m1, m2, 1/rho} read and written
+    bpl = (15 + 2 * 4) * (4 if args.precision == 32 else 8)
     res = {
         "workload": f"{n}^3 RSA pack R {R:g} phi {phi_got:.4f} ({nsph} spheres), flow solved "
                     f"to residual {r.value:.2e} in {steps} steps, scaled to Pe {peclet:g} "
@@ -177,6 +257,8 @@ def mp_workload(L, args, n=128, R=6.0, phi=0.25, peclet=10.0, alpha=0.0, pads=0.
         "algorithmic_bytes_per_node": bpl,
         "achieved_GBs": round(bpl * N / (ms / args.mp_steps * 1e-3) / 1e9, 1),
     }
+    if not args.no_cpu_baseline:
+        res["cpu_baseline"] = cpu_mp_baseline(L, h, n, args.cpu_mp_steps)
     log(f"mp: {res}")
     L.slb_destroy(h)
     return res
@@ -309,6 +391,8 @@ def batch_workload(L, args, u0=1e-3, tau=1.0):
         "translational_drag_over_stokes_median": float(np.median(drag)),
         "resistance_asymmetry_median": float(np.median(asym)),
     }
+    if not args.no_cpu_baseline:
+        res["cpu_baseline"] = cpu_batch_baseline(boxes, args.cpu_batch_steps, u0, tau)
     log(f"batch: {res}")
     return res
 
@@ -545,6 +629,8 @@ def main():
     ap.add_argument("--force", type=float, default=1e-6)
     ap.add_argument("--precision", type=int, default=32, choices=(32, 64))
     ap.add_argument("--cpu-steps", type=int, default=4)
+    ap.add_argument("--cpu-mp-steps", type=int, default=100)
+    ap.add_argument("--cpu-batch-steps", type=int, default=100)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--converge", action="store_true",
                     help="also run FP64 and FP32 to the velocity residual tolerance and "
@@ -659,6 +745,11 @@ def main():
         "cpu_baseline": cpu,
     }
     if conv is not None:
+        if cpu is not None:
+            # the reference's FP32 step time on this pack times the GPU FP32
+            # run's step count: extrapolated, not run
+            conv["cpu_fp32_seconds_to_residual_extrapolated"] = round(
+                conv["fp32"]["steps"] * N / (cpu["value"] * 1e6), 1)
         out["converge"] = conv
     if args.mp:
         out["mp"] = mp_workload(L, args)
