@@ -181,22 +181,34 @@ def converge(L, args):
     from measure() at the end of each run."""
     res = {"tol": args.tol, "interval": args.interval}
     eta = 72.0 * (0.75 - 0.5) / 3.0
-    for prec in (64, 32):
+    for prec, window in ((64, 0), (32, 0), (64, args.aa_window), (32, args.aa_window)):
         os.environ["SLB_PRECISION"] = str(prec)
         h, _, _ = build_pack(L, args.size, args.radius, args.phi, args.seed, args.force)
         r = C.c_double(0.0)
+        L.slb_residual(h)  # uploads the geometry before the clock starts
         t0 = time.perf_counter()
-        steps = L.slb_run_to_residual(h, args.tol, args.interval, args.maxsteps, C.byref(r))
+        if window > 0:
+            steps = L.slb_run_to_residual_aa(h, args.tol, args.interval, args.maxsteps, window,
+                                             C.byref(r))
+        else:
+            steps = L.slb_run_to_residual(h, args.tol, args.interval, args.maxsteps, C.byref(r))
         L.measure(h)
         dt = time.perf_counter() - t0
         k = h.contents.vmeanx * eta / args.force
-        log(f"converge fp{prec}: {steps} steps to residual {r.value:.3e} in {dt:.1f} s, "
+        tag = f"fp{prec}" + (f"_anderson{window}" if window > 0 else "")
+        log(f"converge {tag}: {steps} steps to residual {r.value:.3e} in {dt:.2f} s, "
             f"k = {k:.10g}")
-        res[f"fp{prec}"] = {"steps": steps, "residual": r.value, "seconds": round(dt, 3),
-                            "permeability": k, "vmeanx": h.contents.vmeanx}
+        res[tag] = {"steps": steps, "residual": r.value, "seconds": round(dt, 3),
+                    "permeability": k, "vmeanx": h.contents.vmeanx}
         L.slb_destroy(h)
     k64, k32 = res["fp64"]["permeability"], res["fp32"]["permeability"]
     res["permeability_rel_diff_fp32_vs_fp64"] = abs(k32 - k64) / abs(k64)
+    if args.aa_window > 0:
+        for prec in (64, 32):
+            a, p_ = res[f"fp{prec}_anderson{args.aa_window}"], res[f"fp{prec}"]
+            a["permeability_rel_diff_vs_fp64_plain"] = abs(a["permeability"] - k64) / abs(k64)
+            a["wall_time_plain_over_anderson"] = round(p_["seconds"] / a["seconds"], 3)
+            a["steps_plain_over_anderson"] = round(p_["steps"] / a["steps"], 3)
     return res
 
 
@@ -528,7 +540,10 @@ def warm_workload(L, args):
         for (solid, phi), p in zip(warm_maps(L, size, params, workers), params):
             row = {"n": size, "R": round(p[0], 3), "phi": round(phi, 4), "overlap": p[2]}
             first = len([r for r in rows if r["n"] == size]) < 4
-            for mode in ("cold", "warm", "warm_trace", "equilibrium_trace"):
+            modes = ["cold", "warm", "warm_trace", "equilibrium_trace"]
+            if args.aa_window > 0:
+                modes += ["cold_aa", "warm_aa"]
+            for mode in modes:
                 if mode == "equilibrium_trace" and not first:
                     continue
                 h = warm_lattice(L, size, solid, F)
@@ -544,6 +559,18 @@ def warm_workload(L, args):
                     steps = L.slb_warm_solve(net, h, args.tol, args.interval, args.maxsteps,
                                              C.byref(rr), C.byref(uw))
                     r, used = rr.value, uw.value
+                elif mode in ("cold_aa", "warm_aa"):
+                    # Anderson-mixed solve from rest, or from the warm start
+                    # with no safeguard
+                    used = 0
+                    if mode == "warm_aa":
+                        L.slb_warm_set_mode(net, 1)
+                        L.slb_warm_start(net, h)
+                        used = 1
+                    rr = C.c_double(0.0)
+                    steps = L.slb_run_to_residual_aa(h, args.tol, args.interval, args.maxsteps,
+                                                     args.aa_window, C.byref(rr))
+                    r = rr.value
                 else:
                     # the warm start without the fallback, for its residual history
                     L.slb_warm_set_mode(net, 1 if mode == "warm_trace" else 0)
@@ -612,6 +639,21 @@ def warm_workload(L, args):
                                         for t in ("1e-03", "1e-04", "1e-05", "1e-06")} if eq else None,
             "pred_rel_mse_mean": float(np.mean([r["pred_rel_mse"] for r in rs])),
         }
+        if args.aa_window > 0:
+            def tot(mode, key):
+                return float(sum(r[mode][key] for r in rs))
+            dka = np.array([abs(r[m]["permeability"] - r["cold"]["permeability"])
+                            / abs(r["cold"]["permeability"]) for r in rs
+                            for m in ("cold_aa", "warm_aa")])
+            summ[str(size)]["anderson"] = {
+                "window": args.aa_window,
+                "steps_cold_over_cold_aa": tot("cold", "steps") / tot("cold_aa", "steps"),
+                "wall_cold_over_cold_aa": tot("cold", "seconds") / tot("cold_aa", "seconds"),
+                "steps_cold_over_warm_aa": tot("cold", "steps") / tot("warm_aa", "steps"),
+                "wall_cold_over_warm_aa": tot("cold", "seconds") / tot("warm_aa", "seconds"),
+                "steps_cold_aa_over_warm_aa": tot("cold_aa", "steps") / tot("warm_aa", "steps"),
+                "k_rel_diff_max": float(dka.max()),
+            }
     res["eval"] = summ
     res["eval_rows"] = rows
     return res
@@ -638,6 +680,8 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--interval", type=int, default=100)
     ap.add_argument("--maxsteps", type=int, default=200000)
+    ap.add_argument("--aa-window", type=int, default=5,
+                    help="Anderson window of the accelerated solves in --converge (0: none)")
     ap.add_argument("--mp", action="store_true",
                     help="also run the passive-scalar workload: 128^3 pack (R 6, phi 0.25) "
                          "at Peclet 10, adsorption 0.01, 5000 moment-propagation steps")

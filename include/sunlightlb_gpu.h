@@ -60,6 +60,13 @@ void   slb_residual_reset(LBSys lb);
  * steps were taken.  Returns the steps taken; *res receives the last
  * residual. */
 int slb_run_to_residual(LBSys lb, double tol, int interval, int maxsteps, double *res);
+/* The same solve with Anderson mixing over the last `window` (1..16)
+ * iterations of `interval` steps each: the next state is the combination of
+ * recent iterates that minimises the fixed-point residual.  Ends, as above,
+ * in a state whose plain `interval`-step residual is below tol.  Needs
+ * 2 window + 3 extra population arrays on the device.  window 0: plain. */
+int slb_run_to_residual_aa(LBSys lb, double tol, int interval, int maxsteps, int window,
+                           double *res);
 
 /* Time nsteps fused steps on the device between HIP events (the state
  * advances).  Returns milliseconds. */

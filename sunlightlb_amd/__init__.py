@@ -152,6 +152,7 @@ SIGNATURES = {
     "slb_run": (None, [LBSys, I]), "slb_residual": (D, [LBSys]),
     "slb_residual_reset": (None, [LBSys]),
     "slb_run_to_residual": (I, [LBSys, D, I, I, RealP]),
+    "slb_run_to_residual_aa": (I, [LBSys, D, I, I, I, RealP]),
     "slb_time_steps": (D, [LBSys, I]), "slb_scale_pops": (None, [LBSys, D]),
     "slb_sum_pops": (None, [LBSys, RealP]),
     "slb_insert_spheres": (I, [LBSys, I, RealP]),

@@ -8,6 +8,7 @@
  * makemask, syncobjs, countfluid, countmask, vspread, zerobf, addbf, vset
  * (src/lbobjs.c:526-714).
  */
+#include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -607,6 +608,113 @@ int slb_run_to_residual(LBSys lb, double tol, int interval, int maxsteps, double
     r = slb_residual(lb);
     if (r >= 0.0 && r < tol) break;
   }
+  if (res) *res = r;
+  return steps;
+}
+
+/* Solve of a small dense system (k <= SLBD_VEC_MAX) by Gaussian elimination
+ * with partial pivoting; returns -1 if singular. */
+static int small_solve(int k, double *A, double *b) {
+  for (int c = 0; c < k; c++) {
+    int piv = c;
+    for (int r = c + 1; r < k; r++)
+      if (fabs(A[r * k + c]) > fabs(A[piv * k + c])) piv = r;
+    if (!(fabs(A[piv * k + c]) > 0.0)) return -1;
+    if (piv != c) {
+      for (int q = 0; q < k; q++) { double t = A[c * k + q]; A[c * k + q] = A[piv * k + q]; A[piv * k + q] = t; }
+      double t = b[c]; b[c] = b[piv]; b[piv] = t;
+    }
+    for (int r = c + 1; r < k; r++) {
+      const double f = A[r * k + c] / A[c * k + c];
+      for (int q = c; q < k; q++) A[r * k + q] -= f * A[c * k + q];
+      b[r] -= f * b[c];
+    }
+  }
+  for (int r = k - 1; r >= 0; r--) {
+    double t = b[r];
+    for (int q = r + 1; q < k; q++) t -= A[r * k + q] * b[q];
+    b[r] = t / A[r * k + r];
+  }
+  return 0;
+}
+
+/* Run to the velocity residual with Anderson mixing of the steady-state
+ * iteration.  One iteration is G = `interval` ordinary steps from state x_i;
+ * the residual of G(x_i) against x_i is the residual slb_run_to_residual
+ * tests, and the solve ends, in the state G(x_i), the first time it is below
+ * tol.  Otherwise the next state is
+ *   x_{i+1} = G(x_i) - sum_j gamma_j dG_j,
+ * gamma minimising |f_i - sum_j gamma_j dF_j| over the last `window`
+ * differences dF_j = f_{j+1} - f_j, dG_j = G(x_{j+1}) - G(x_j) of the
+ * fixed-point residuals f = G(x) - x (Anderson's type II mixing, equivalent
+ * to GMRES for the linear Stokes iteration).  The normal equations are
+ * solved in FP64 with a 1e-10 relative ridge; the history restarts when the
+ * residual grows tenfold over the best one.  Memory: 2 window + 3 population
+ * arrays on the device. */
+int slb_run_to_residual_aa(LBSys lb, double tol, int interval, int maxsteps, int window,
+                           double *res) {
+  if (window < 1) return slb_run_to_residual(lb, tol, interval, maxsteps, res);
+  if (window > SLBD_VEC_MAX) window = SLBD_VEC_MAX;
+  if (interval < 1) interval = 1;
+  slb_lbpriv *S = PRIV(lb);
+  slbd_lattice *D = S->dev;
+  slb_dev_geom(lb);
+  slb_dev_pops(lb);
+  const int m = window;
+  void *base = slbd_vec_alloc(D, 3 + 2 * m);
+  void *F = slbd_vec_at(D, base, 0), *Fp = slbd_vec_at(D, base, 1);
+  void *Gp = slbd_vec_at(D, base, 2);
+  void *dF[SLBD_VEC_MAX], *dG[SLBD_VEC_MAX];
+  for (int j = 0; j < m; j++) {
+    dF[j] = slbd_vec_at(D, base, 3 + j);
+    dG[j] = slbd_vec_at(D, base, 3 + m + j);
+  }
+  double gram[SLBD_VEC_MAX * SLBD_VEC_MAX], A[SLBD_VEC_MAX * SLBD_VEC_MAX];
+  double rhs[SLBD_VEC_MAX], row[SLBD_VEC_MAX];
+  int k = 0, head = 0, have_prev = 0, steps = 0;
+  double r = -1.0, best = HUGE_VAL;
+  slb_residual_reset(lb);
+  slb_residual(lb);
+  while (steps < maxsteps) {
+    int n = interval;
+    if (steps + n > maxsteps) n = maxsteps - steps;
+    slbd_vec_copy(D, F, slbd_vec_cur(D));      /* x_i */
+    slb_run(lb, n);
+    steps += n;
+    r = slb_residual(lb);
+    if (r >= 0.0 && r < tol) break;
+    if (r > 10.0 * best) { k = 0; head = 0; have_prev = 0; }   /* restart the history */
+    if (r >= 0.0 && r < best) best = r;
+    void *g = slbd_vec_cur(D);
+    slbd_vec_sub(D, F, g, F);                  /* f_i = G(x_i) - x_i */
+    if (have_prev) {
+      const int slot = head;
+      slbd_vec_sub2(D, dF[slot], F, Fp, dG[slot], g, Gp);
+      if (k < m) k++;
+      head = (head + 1) % m;
+      /* new Gram row and right-hand side in one pass over the dF columns */
+      slbd_vec_dots2(D, dF[slot], F, dF, k, row, rhs);
+      for (int j = 0; j < k; j++) { gram[slot * m + j] = row[j]; gram[j * m + slot] = row[j]; }
+    }
+    { void *t = Fp; Fp = F; F = t; }          /* keep f_i; F is free for x_{i+1} */
+    slbd_vec_copy(D, Gp, g);
+    have_prev = 1;
+    if (k > 0) {
+      double dmax = 0.0;
+      for (int j = 0; j < k; j++) dmax = gram[j * m + j] > dmax ? gram[j * m + j] : dmax;
+      for (int a = 0; a < k; a++)
+        for (int b = 0; b < k; b++) A[a * k + b] = gram[a * m + b] + (a == b ? 1e-10 * dmax : 0.0);
+      if (small_solve(k, A, rhs) == 0) {
+        slbd_vec_mix(D, g, g, dG, rhs, k);     /* x_{i+1}, in place */
+        S->pops_loc = LOC_DEV;
+      } else {
+        k = 0; head = 0;
+      }
+    }
+    slb_residual_reset(lb);
+    slb_residual(lb);
+  }
+  slbd_vec_free(base);
   if (res) *res = r;
   return steps;
 }

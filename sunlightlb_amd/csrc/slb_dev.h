@@ -147,6 +147,27 @@ void slbd_moments(slbd_lattice *L, const slbd_params *P, const float *sc, float 
 void slbd_inject(slbd_lattice *L, const slbd_params *P, const float *U, int CU,
                  const double *inv_sc, int mode);
 
+/* ---- population vectors for Anderson mixing (whole population arrays in
+ * the lattice precision, on the lattice's stream) ---- */
+#define SLBD_VEC_MAX 16
+void *slbd_vec_cur(slbd_lattice *L);                 /* current populations */
+void *slbd_vec_alloc(slbd_lattice *L, int count);    /* count vectors */
+void *slbd_vec_at(slbd_lattice *L, void *base, int k);
+void  slbd_vec_free(void *p);
+void  slbd_vec_copy(slbd_lattice *L, void *dst, const void *src);
+void  slbd_vec_sub(slbd_lattice *L, void *z, const void *x, const void *y);   /* z = x - y */
+/* out[j] = a . B[j], FP64, j < m <= SLBD_VEC_MAX, fixed summation order */
+void  slbd_vec_dots(slbd_lattice *L, const void *a, void *const *B, int m, double *out);
+/* out[j] = a . B[j] and out2[j] = a2 . B[j] in one pass over B */
+void  slbd_vec_dots2(slbd_lattice *L, const void *a, const void *a2, void *const *B, int m,
+                     double *out, double *out2);
+/* z1 = x1 - y1 and z2 = x2 - y2 in one pass */
+void  slbd_vec_sub2(slbd_lattice *L, void *z1, const void *x1, const void *y1, void *z2,
+                    const void *x2, const void *y2);
+/* z = g - sum_j gamma[j] B[j] */
+void  slbd_vec_mix(slbd_lattice *L, void *z, const void *g, void *const *B,
+                   const double *gamma, int m);
+
 /* ---- batched independent lattices (slb_batch.hip) ---- */
 typedef struct slbd_batch slbd_batch;
 slbd_batch *slbd_batch_create(int prec);

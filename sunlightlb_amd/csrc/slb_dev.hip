@@ -1067,6 +1067,197 @@ extern "C" void slbd_scale_pops(slbd_lattice *L, double s) {
   LAUNCHCHECK();
 }
 
+/* ---------------------------------------------------------------------- */
+/* Population vectors (whole arrays of 15 pop_nodes(N) values in the        */
+/* lattice precision) for Anderson mixing of the steady-state iteration.   */
+
+struct VecSet {
+  const void *p[SLBD_VEC_MAX];
+  double g[SLBD_VEC_MAX];
+  int m;
+};
+
+template <typename T>
+__global__ void k_vsub(T *__restrict__ z, const T *__restrict__ x, const T *__restrict__ y,
+                       long n) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long)gridDim.x * blockDim.x)
+    z[i] = x[i] - y[i];
+}
+
+/* z1 = x1 - y1, z2 = x2 - y2 in one pass */
+template <typename T>
+__global__ void k_vsub2(T *__restrict__ z1, const T *__restrict__ x1, const T *__restrict__ y1,
+                        T *__restrict__ z2, const T *__restrict__ x2, const T *__restrict__ y2,
+                        long n) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long)gridDim.x * blockDim.x) {
+    z1[i] = x1[i] - y1[i];
+    z2[i] = x2[i] - y2[i];
+  }
+}
+
+/* part[b * SLBD_VEC_MAX + j] = block b's share of a . B_j (FP64); with a2,
+ * part[(gridDim + b) * SLBD_VEC_MAX + j] = block b's share of a2 . B_j, each
+ * B_j read once for both */
+template <typename T, bool TWO>
+__global__ void __launch_bounds__(256)
+k_vdots(const T *__restrict__ a, const T *__restrict__ a2, VecSet B, long n,
+        double *__restrict__ part) {
+  __shared__ double red[256];
+  double acc[SLBD_VEC_MAX], acc2[TWO ? SLBD_VEC_MAX : 1];
+#pragma unroll
+  for (int j = 0; j < SLBD_VEC_MAX; j++) acc[j] = 0.0;
+#pragma unroll
+  for (int j = 0; j < (TWO ? SLBD_VEC_MAX : 1); j++) acc2[j] = 0.0;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long)gridDim.x * blockDim.x) {
+    const double ai = (double)a[i];
+    const double bi = TWO ? (double)a2[i] : 0.0;
+#pragma unroll
+    for (int j = 0; j < SLBD_VEC_MAX; j++)
+      if (j < B.m) {
+        const double v = (double)((const T *)B.p[j])[i];
+        acc[j] += ai * v;
+        if (TWO) acc2[j] += bi * v;
+      }
+  }
+  for (int q = 0; q < (TWO ? 2 : 1); q++)
+    for (int j = 0; j < B.m; j++) {
+      red[threadIdx.x] = q ? acc2[TWO ? j : 0] : acc[j];
+      __syncthreads();
+      for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+      }
+      if (threadIdx.x == 0) part[((long)q * gridDim.x + blockIdx.x) * SLBD_VEC_MAX + j] = red[0];
+      __syncthreads();
+    }
+}
+
+/* z = g - sum_j gamma_j B_j (z may be g) */
+template <typename T>
+__global__ void k_vmix(T *z, const T *g, VecSet B, long n) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long)gridDim.x * blockDim.x) {
+    double v = (double)g[i];
+    for (int j = 0; j < B.m; j++) v -= B.g[j] * (double)((const T *)B.p[j])[i];
+    z[i] = (T)v;
+  }
+}
+
+static long vec_len(const slbd_lattice *L) { return 15 * pop_nodes(L->N); }
+static dim3 vec_grid(long n) {
+  const long b = (n + 255) / 256;
+  return dim3((unsigned)(b < RED_BLOCKS ? b : RED_BLOCKS));
+}
+
+extern "C" void *slbd_vec_cur(slbd_lattice *L) { return L->pops[L->cur]; }
+
+extern "C" void *slbd_vec_alloc(slbd_lattice *L, int count) {
+  return dalloc((size_t)count * vec_len(L) * tsize(L->prec));
+}
+
+extern "C" void *slbd_vec_at(slbd_lattice *L, void *base, int k) {
+  return (char *)base + (size_t)k * vec_len(L) * tsize(L->prec);
+}
+
+extern "C" void slbd_vec_free(void *p) {
+  if (p) HIPCHECK(hipFree(p));
+}
+
+extern "C" void slbd_vec_copy(slbd_lattice *L, void *dst, const void *src) {
+  HIPCHECK(hipMemcpyAsync(dst, src, (size_t)vec_len(L) * tsize(L->prec),
+                          hipMemcpyDeviceToDevice, L->stream));
+}
+
+extern "C" void slbd_vec_sub(slbd_lattice *L, void *z, const void *x, const void *y) {
+  const long n = vec_len(L);
+  if (L->prec == 32)
+    hipLaunchKernelGGL(k_vsub<float>, vec_grid(n), dim3(256), 0, L->stream, (float *)z,
+                       (const float *)x, (const float *)y, n);
+  else
+    hipLaunchKernelGGL(k_vsub<double>, vec_grid(n), dim3(256), 0, L->stream, (double *)z,
+                       (const double *)x, (const double *)y, n);
+  LAUNCHCHECK();
+}
+
+extern "C" void slbd_vec_dots2(slbd_lattice *L, const void *a, const void *a2, void *const *B,
+                               int m, double *out, double *out2) {
+  if (m < 1 || m > SLBD_VEC_MAX) return;
+  VecSet S;
+  memset(&S, 0, sizeof S);
+  S.m = m;
+  for (int j = 0; j < m; j++) S.p[j] = B[j];
+  const long n = vec_len(L);
+  /* two partial sets must fit L->part (RED_BLOCKS x 16 doubles) */
+  dim3 grid = vec_grid(n);
+  if (a2 && grid.x > RED_BLOCKS / 2) grid.x = RED_BLOCKS / 2;
+  if (L->prec == 32) {
+    if (a2)
+      hipLaunchKernelGGL((k_vdots<float, true>), grid, dim3(256), 0, L->stream,
+                         (const float *)a, (const float *)a2, S, n, L->part);
+    else
+      hipLaunchKernelGGL((k_vdots<float, false>), grid, dim3(256), 0, L->stream,
+                         (const float *)a, (const float *)nullptr, S, n, L->part);
+  } else {
+    if (a2)
+      hipLaunchKernelGGL((k_vdots<double, true>), grid, dim3(256), 0, L->stream,
+                         (const double *)a, (const double *)a2, S, n, L->part);
+    else
+      hipLaunchKernelGGL((k_vdots<double, false>), grid, dim3(256), 0, L->stream,
+                         (const double *)a, (const double *)nullptr, S, n, L->part);
+  }
+  LAUNCHCHECK();
+  static double host[RED_BLOCKS * SLBD_VEC_MAX];
+  const int sets = a2 ? 2 : 1;
+  HIPCHECK(hipMemcpyAsync(host, L->part, sizeof(double) * sets * grid.x * SLBD_VEC_MAX,
+                          hipMemcpyDeviceToHost, L->stream));
+  HIPCHECK(hipStreamSynchronize(L->stream));
+  for (int q = 0; q < sets; q++)
+    for (int j = 0; j < m; j++) {
+      double t = 0.0;
+      for (unsigned b = 0; b < grid.x; b++) t += host[((long)q * grid.x + b) * SLBD_VEC_MAX + j];
+      (q ? out2 : out)[j] = t;
+    }
+}
+
+extern "C" void slbd_vec_dots(slbd_lattice *L, const void *a, void *const *B, int m,
+                              double *out) {
+  slbd_vec_dots2(L, a, nullptr, B, m, out, nullptr);
+}
+
+extern "C" void slbd_vec_sub2(slbd_lattice *L, void *z1, const void *x1, const void *y1,
+                              void *z2, const void *x2, const void *y2) {
+  const long n = vec_len(L);
+  if (L->prec == 32)
+    hipLaunchKernelGGL(k_vsub2<float>, vec_grid(n), dim3(256), 0, L->stream, (float *)z1,
+                       (const float *)x1, (const float *)y1, (float *)z2, (const float *)x2,
+                       (const float *)y2, n);
+  else
+    hipLaunchKernelGGL(k_vsub2<double>, vec_grid(n), dim3(256), 0, L->stream, (double *)z1,
+                       (const double *)x1, (const double *)y1, (double *)z2,
+                       (const double *)x2, (const double *)y2, n);
+  LAUNCHCHECK();
+}
+
+extern "C" void slbd_vec_mix(slbd_lattice *L, void *z, const void *g, void *const *B,
+                             const double *gamma, int m) {
+  if (m < 0 || m > SLBD_VEC_MAX) return;
+  VecSet S;
+  memset(&S, 0, sizeof S);
+  S.m = m;
+  for (int j = 0; j < m; j++) { S.p[j] = B[j]; S.g[j] = gamma[j]; }
+  const long n = vec_len(L);
+  if (L->prec == 32)
+    hipLaunchKernelGGL(k_vmix<float>, vec_grid(n), dim3(256), 0, L->stream, (float *)z,
+                       (const float *)g, S, n);
+  else
+    hipLaunchKernelGGL(k_vmix<double>, vec_grid(n), dim3(256), 0, L->stream, (double *)z,
+                       (const double *)g, S, n);
+  LAUNCHCHECK();
+}
+
 static PhysArgs phys(const slbd_params *P) {
   PhysArgs a;
   a.omega = 1.0 / P->tau;
