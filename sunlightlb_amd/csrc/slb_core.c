@@ -683,6 +683,7 @@ int slb_run_to_residual_aa(LBSys lb, double tol, int interval, int maxsteps, int
     steps += n;
     r = slb_residual(lb);
     if (r >= 0.0 && r < tol) break;
+    if (steps >= maxsteps) break;              /* end on an iterate G(x_i), not a mix */
     if (r > 10.0 * best) { k = 0; head = 0; have_prev = 0; }   /* restart the history */
     if (r >= 0.0 && r < best) best = r;
     void *g = slbd_vec_cur(D);
