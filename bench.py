@@ -609,6 +609,8 @@ def warm_workload(L, args):
     summ = {}
     for size in sizes:
         rs = [r for r in rows if r["n"] == size]
+        if not rs:
+            continue
         cs = np.array([r["cold"]["seconds"] for r in rs])
         ws = np.array([r["warm"]["seconds"] for r in rs])
         cst = np.array([r["cold"]["steps"] for r in rs])

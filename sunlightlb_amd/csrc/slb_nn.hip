@@ -71,65 +71,89 @@ __device__ __forceinline__ int tap_node(const NGeom &G, int z, int y, int x, int
 /* ---------------------------------------------------------------------- */
 /* Forward / grad-input convolution (same kernel, tap direction sgn).       */
 /* Y[n][co] = act( bias[co] + sum_k W[co][k] * X[n + sgn*off(tap(k))][ci(k)] ) */
-/* Block: 256 threads = 4 waves; wave w handles nodes n0 + 16w .. +15 and all  */
-/* output channels (MT = Cout/16 tiles).                                     */
+/* Block: 256 threads = 4 waves; wave w handles NTW tiles of 16 consecutive   */
+/* nodes and all output channels (MT = Cout/16 tiles); each K chunk's         */
+/* weights are loaded once and applied to all NTW tiles, whose NTW input     */
+/* loads are in flight together.                                            */
 
-template <int MT>
+template <int MT, int NTW>
 __global__ void __launch_bounds__(256)
 k_conv(const uint16_t *__restrict__ X, int Cin, const uint16_t *__restrict__ W, int Kp,
        const float *__restrict__ bias, uint16_t *__restrict__ Y, float *__restrict__ Yf,
        int Cout, int relu, int sgn, NGeom G) {
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int n0 = (blockIdx.x * 4 + wv) * 16;
+  const int n0 = (blockIdx.x * 4 + wv) * 16 * NTW;
   if (n0 >= G.N) return;
-  const int n = n0 + (lane & 15);
   const int grp = lane >> 4;
-  const bool valid = n < G.N;
-  const int nn = valid ? n : n0;
-  const int z = nn / G.plane, rem = nn - z * G.plane, y = rem / G.Wd, x = rem - y * G.Wd;
-  const int K = 27 * Cin;
-  f32x4 acc[MT];
+  int zc[NTW], yc[NTW], xc[NTW];
 #pragma unroll
-  for (int m = 0; m < MT; m++) acc[m] = (f32x4){0.f, 0.f, 0.f, 0.f};
+  for (int u = 0; u < NTW; u++) {
+    int nn = n0 + 16 * u + (lane & 15);
+    if (nn >= G.N) nn = n0;
+    zc[u] = nn / G.plane;
+    const int rem = nn - zc[u] * G.plane;
+    yc[u] = rem / G.Wd;
+    xc[u] = rem - yc[u] * G.Wd;
+  }
+  const int K = 27 * Cin;
+  f32x4 acc[NTW][MT];
+#pragma unroll
+  for (int u = 0; u < NTW; u++)
+#pragma unroll
+    for (int m = 0; m < MT; m++) acc[u][m] = (f32x4){0.f, 0.f, 0.f, 0.f};
   for (int k0 = 0; k0 < Kp; k0 += 32) {
     const int kk = k0 + 8 * grp;
-    bf16x8 b;
-    if (kk < K) {
-      const int t = kk / Cin, ci = kk - t * Cin;
-      const int src = tap_node(G, z, y, x, t, sgn);
-      b = *(const bf16x8 *)(X + (long)src * Cin + ci);
-    } else {
-      b = (bf16x8){0, 0, 0, 0, 0, 0, 0, 0};
-    }
+    /* the weights of this K chunk, loaded once for all NTW node tiles */
+    bf16x8 a[MT];
 #pragma unroll
     for (int m = 0; m < MT; m++) {
       const int co = 16 * m + (lane & 15);
-      bf16x8 a = (bf16x8){0, 0, 0, 0, 0, 0, 0, 0};
-      if (co < Cout) a = *(const bf16x8 *)(W + (long)co * Kp + kk);
-      acc[m] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[m], 0, 0, 0);
+      a[m] = (bf16x8){0, 0, 0, 0, 0, 0, 0, 0};
+      if (co < Cout) a[m] = *(const bf16x8 *)(W + (long)co * Kp + kk);
     }
+    bf16x8 b[NTW];
+    if (kk < K) {
+      const int t = kk / Cin, ci = kk - t * Cin;
+#pragma unroll
+      for (int u = 0; u < NTW; u++) {
+        const int src = tap_node(G, zc[u], yc[u], xc[u], t, sgn);
+        b[u] = *(const bf16x8 *)(X + (long)src * Cin + ci);
+      }
+    } else {
+#pragma unroll
+      for (int u = 0; u < NTW; u++) b[u] = (bf16x8){0, 0, 0, 0, 0, 0, 0, 0};
+    }
+#pragma unroll
+    for (int u = 0; u < NTW; u++)
+#pragma unroll
+      for (int m = 0; m < MT; m++)
+        acc[u][m] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[m], b[u], acc[u][m], 0, 0, 0);
   }
-  if (!valid) return;
-  /* lane holds D[co = 16m + 4*grp + r][node n] */
+  /* lane holds D[co = 16m + 4*grp + r][node n0 + 16u + lane%16] */
 #pragma unroll
-  for (int m = 0; m < MT; m++) {
-    const int cb = 16 * m + 4 * grp;
-    if (cb >= Cout) continue;
-    float v[4];
+  for (int u = 0; u < NTW; u++) {
+    const int n = n0 + 16 * u + (lane & 15);
+    if (n >= G.N) continue;
 #pragma unroll
-    for (int r = 0; r < 4; r++) {
-      v[r] = acc[m][r] + (bias ? bias[cb + r] : 0.f);
-      if (relu) v[r] = v[r] > 0.f ? v[r] : 0.f;
-    }
-    if (Y) {
-      uint2 pk;
-      pk.x = (uint32_t)f2bf(v[0]) | ((uint32_t)f2bf(v[1]) << 16);
-      pk.y = (uint32_t)f2bf(v[2]) | ((uint32_t)f2bf(v[3]) << 16);
-      *(uint2 *)(Y + (long)n * Cout + cb) = pk;
-    }
-    if (Yf) {
+    for (int m = 0; m < MT; m++) {
+      const int cb = 16 * m + 4 * grp;
+      if (cb >= Cout) continue;
+      float v[4];
 #pragma unroll
-      for (int r = 0; r < 4; r++) Yf[(long)n * Cout + cb + r] = v[r];
+      for (int r = 0; r < 4; r++) {
+        v[r] = acc[u][m][r] + (bias ? bias[cb + r] : 0.f);
+        if (relu) v[r] = v[r] > 0.f ? v[r] : 0.f;
+      }
+      if (Y) {
+        uint2 pk;
+        pk.x = (uint32_t)f2bf(v[0]) | ((uint32_t)f2bf(v[1]) << 16);
+        pk.y = (uint32_t)f2bf(v[2]) | ((uint32_t)f2bf(v[3]) << 16);
+        *(uint2 *)(Y + (long)n * Cout + cb) = pk;
+      }
+      if (Yf) {
+#pragma unroll
+        for (int r = 0; r < 4; r++) Yf[(long)n * Cout + cb + r] = v[r];
+      }
     }
   }
 }
@@ -500,6 +524,12 @@ static NGeom ngeom(int D, int H, int Wd) {
   return g;
 }
 
+/* node tiles of 16 per wave in the convolution (weights of a K chunk are
+ * loaded once for all of them) */
+#ifndef SLBN_CONV_NTW
+#define SLBN_CONV_NTW 4
+#endif
+
 static hipStream_t nn_stream = 0;
 extern "C" void slbn_set_stream(void *s) { nn_stream = (hipStream_t)s; }
 
@@ -512,14 +542,16 @@ extern "C" void slbn_conv(const uint16_t *X, int cin, const uint16_t *W, const f
                           int Wd) {
   const NGeom G = ngeom(D, H, Wd);
   const int Kp = slbn_kpad(cin);
-  const unsigned g = blocks_for(G.N, 64);
+  /* 4 waves per block, each NTW node tiles of 16 */
+  const unsigned g = blocks_for(G.N, 64 * SLBN_CONV_NTW);
   const int sgn = transposed ? -1 : 1;
   if (cin < 8 || cin % 8 || cout < 1 || cout % 4 || (long)G.N * (cin > cout ? cin : cout) >= (1L << 31)) {
     fprintf(stderr, "slbn_conv: cin must be a multiple of 8, cout of 4, tensors < 2^31 elements\n");
     exit(EXIT_FAILURE);
   }
 #define SLBN_CONV(MT)                                                                      \
-  hipLaunchKernelGGL(k_conv<MT>, dim3(g), dim3(256), 0, nn_stream, X, cin, W, Kp, bias, Y, Yf, \
+  hipLaunchKernelGGL((k_conv<MT, SLBN_CONV_NTW>), dim3(g), dim3(256), 0, nn_stream, X, cin, W, \
+                     Kp, bias, Y, Yf, \
                      cout, relu, sgn, G)
   const int mt = (cout + 15) / 16;
   if (mt == 1) SLBN_CONV(1);
