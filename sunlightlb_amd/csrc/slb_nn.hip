@@ -253,29 +253,36 @@ typedef __attribute__((ext_vector_type(4))) uint32_t u32x4;
 
 __device__ __forceinline__ bf16x8 as_bf(u32x4 v) { return __builtin_bit_cast(bf16x8, v); }
 
-template <int MT, int NT>
+template <int MT, int NT, int TX>
 __global__ void __launch_bounds__(256)
 k_wgrad_b(const uint16_t *__restrict__ Xb, const uint16_t *__restrict__ Db,
           float *__restrict__ part, int gpb, NGeom G) {
+  /* TX = 1: blockIdx.y is the tap; TX = 3: blockIdx.y is the (dz, dy) pair
+   * and the block computes the three taps dx = -1, 0, +1 from one load of
+   * the output gradient and of the input groups either side */
   constexpr int Cin = 16 * NT, Cout = 16 * MT;
-  __shared__ float red[3][MT * NT * 4][64];
+  __shared__ float red[3][TX * MT * NT * 4][64];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, grp = lane >> 4, m = lane & 15;
-  const int t = blockIdx.y;
-  const int dz = t / 9 - 1, dy = (t / 3) % 3 - 1, dx = t % 3 - 1;
+  const int tq = blockIdx.y;
+  const int dz = TX == 3 ? tq / 3 - 1 : tq / 9 - 1;
+  const int dy = TX == 3 ? tq % 3 - 1 : (tq / 3) % 3 - 1;
+  const int dx1 = TX == 3 ? 0 : tq % 3 - 1;   /* the single dx of TX = 1 */
   const int gW = G.Wd >> 3;
   const long ngr = (long)G.N >> 3;
   const long gb = (long)blockIdx.x * gpb;
   const long ge = min(ngr, gb + gpb);
-  f32x4 acc[MT][NT];
+  f32x4 acc[TX][MT][NT];
 #pragma unroll
-  for (int i = 0; i < MT; i++)
+  for (int d = 0; d < TX; d++)
 #pragma unroll
-    for (int j = 0; j < NT; j++) acc[i][j] = (f32x4){0.f, 0.f, 0.f, 0.f};
+    for (int i = 0; i < MT; i++)
+#pragma unroll
+      for (int j = 0; j < NT; j++) acc[d][i][j] = (f32x4){0.f, 0.f, 0.f, 0.f};
   for (long g0 = gb + 4 * wv; g0 < ge; g0 += 16) {
     const long g = g0 + grp;
     const bool ok = g < ge;
     const long gg = ok ? g : gb;
-    bf16x8 a[MT], b[NT];
+    bf16x8 a[MT];
 #pragma unroll
     for (int i = 0; i < MT; i++) {
       u32x4 v = *(const u32x4 *)(Db + (gg * Cout + 16 * i + m) * 8);
@@ -289,64 +296,72 @@ k_wgrad_b(const uint16_t *__restrict__ Xb, const uint16_t *__restrict__ Db,
     yy += (yy < 0) ? G.H : 0; yy -= (yy >= G.H) ? G.H : 0;
     const long rb = ((long)zz * G.H + yy) * gW;
     const long s0 = rb + xg;
-    int xn = xg + dx;
-    xn += (xn < 0) ? gW : 0; xn -= (xn >= gW) ? gW : 0;
-    const long s1 = rb + xn;
+    const long sm = rb + (xg > 0 ? xg - 1 : gW - 1);     /* the group at x - 8 */
+    const long sp = rb + (xg < gW - 1 ? xg + 1 : 0);     /* the group at x + 8 */
 #pragma unroll
     for (int j = 0; j < NT; j++) {
       const int ci = 16 * j + m;
-      u32x4 v0 = *(const u32x4 *)(Xb + (s0 * Cin + ci) * 8);
-      u32x4 v;
-      if (dx == 0) {
-        v = v0;
-      } else {
-        const u32x4 v1 = *(const u32x4 *)(Xb + (s1 * Cin + ci) * 8);
-        if (dx > 0) {
-          /* nodes x0+1 .. x0+8: elements 1..7 of v0, element 0 of v1 */
+      const u32x4 v0 = *(const u32x4 *)(Xb + (s0 * Cin + ci) * 8);
+      u32x4 vm = v0, vp = v0;
+      if (TX == 3 || dx1 < 0) vm = *(const u32x4 *)(Xb + (sm * Cin + ci) * 8);
+      if (TX == 3 || dx1 > 0) vp = *(const u32x4 *)(Xb + (sp * Cin + ci) * 8);
+#pragma unroll
+      for (int d = 0; d < TX; d++) {
+        const int dx = TX == 3 ? d - 1 : dx1;
+        u32x4 v;
+        if (dx == 0) {
+          v = v0;
+        } else if (dx > 0) {
+          /* nodes x0+1 .. x0+8: elements 1..7 of v0, element 0 of vp */
           v.x = (v0.x >> 16) | (v0.y << 16);
           v.y = (v0.y >> 16) | (v0.z << 16);
           v.z = (v0.z >> 16) | (v0.w << 16);
-          v.w = (v0.w >> 16) | (v1.x << 16);
+          v.w = (v0.w >> 16) | (vp.x << 16);
         } else {
-          /* nodes x0-1 .. x0+6: element 7 of v1, elements 0..6 of v0 */
-          v.x = (v1.w >> 16) | (v0.x << 16);
+          /* nodes x0-1 .. x0+6: element 7 of vm, elements 0..6 of v0 */
+          v.x = (vm.w >> 16) | (v0.x << 16);
           v.y = (v0.x >> 16) | (v0.y << 16);
           v.z = (v0.y >> 16) | (v0.z << 16);
           v.w = (v0.z >> 16) | (v0.w << 16);
         }
+        if (!ok) v = (u32x4){0u, 0u, 0u, 0u};
+        const bf16x8 bb = as_bf(v);
+#pragma unroll
+        for (int i = 0; i < MT; i++)
+          acc[d][i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], bb, acc[d][i][j], 0, 0, 0);
       }
-      if (!ok) v = (u32x4){0u, 0u, 0u, 0u};
-      b[j] = as_bf(v);
     }
-#pragma unroll
-    for (int i = 0; i < MT; i++)
-#pragma unroll
-      for (int j = 0; j < NT; j++)
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
   }
   /* reduce the 4 waves' partial tiles: waves 1..3 write, wave 0 adds */
   if (wv > 0) {
 #pragma unroll
+    for (int d = 0; d < TX; d++)
+#pragma unroll
+      for (int i = 0; i < MT; i++)
+#pragma unroll
+        for (int j = 0; j < NT; j++)
+#pragma unroll
+          for (int r = 0; r < 4; r++)
+            red[wv - 1][((d * MT + i) * NT + j) * 4 + r][lane] = acc[d][i][j][r];
+  }
+  __syncthreads();
+  if (wv > 0) return;
+#pragma unroll
+  for (int d = 0; d < TX; d++) {
+    const int t = TX == 3 ? 3 * tq + d : tq;
+    float *out = part + ((long)t * gridDim.x + blockIdx.x) * Cout * Cin;
+#pragma unroll
     for (int i = 0; i < MT; i++)
 #pragma unroll
       for (int j = 0; j < NT; j++)
 #pragma unroll
-        for (int r = 0; r < 4; r++) red[wv - 1][(i * NT + j) * 4 + r][lane] = acc[i][j][r];
+        for (int r = 0; r < 4; r++) {
+          const int q = ((d * MT + i) * NT + j) * 4 + r;
+          const float v = acc[d][i][j][r] + red[0][q][lane] + red[1][q][lane] + red[2][q][lane];
+          /* lane holds D[co = 16i + 4 grp + r][ci = 16j + m] */
+          out[(16 * i + 4 * grp + r) * Cin + 16 * j + m] = v;
+        }
   }
-  __syncthreads();
-  if (wv > 0) return;
-  float *out = part + ((long)t * gridDim.x + blockIdx.x) * Cout * Cin;
-#pragma unroll
-  for (int i = 0; i < MT; i++)
-#pragma unroll
-    for (int j = 0; j < NT; j++)
-#pragma unroll
-      for (int r = 0; r < 4; r++) {
-        const int q = (i * NT + j) * 4 + r;
-        const float v = acc[i][j][r] + red[0][q][lane] + red[1][q][lane] + red[2][q][lane];
-        /* lane holds D[co = 16i + 4 grp + r][ci = 16j + m] */
-        out[(16 * i + 4 * grp + r) * Cin + 16 * j + m] = v;
-      }
 }
 
 /* dW[co][t*Cin + ci] += sum over blocks b (in order) of part[t][b][co][ci] */
@@ -619,8 +634,12 @@ extern "C" void slbn_conv_wgrad(const uint16_t *X, int cin, const uint16_t *dY, 
   const long ngr = (long)G.N >> 3;
   /* enough blocks per tap to fill the chip with 27 taps, each block a
    * contiguous group range in steps of 16 groups */
+  const int mt = cout / 16, nt = cin / 16;
+  /* three taps per block where the accumulators fit (MT NT <= 2) */
+  const int tx = (mt * nt <= 2) ? 3 : 1;
+  const long nbmax = tx == 3 ? 192 : 64;
   long nb = (ngr + 255) / 256;
-  if (nb > 64) nb = 64;
+  if (nb > nbmax) nb = nbmax;
   if (nb < 1) nb = 1;
   int gpb = (int)(((ngr + nb - 1) / nb + 15) / 16 * 16);
   nb = (ngr + gpb - 1) / gpb;
@@ -635,19 +654,19 @@ extern "C" void slbn_conv_wgrad(const uint16_t *X, int cin, const uint16_t *dY, 
   hipLaunchKernelGGL(k_block, dim3(blocks_for(ngr * cout, 256)), dim3(256), 0, nn_stream, dY, cout,
                      Db, (long)G.N);
   LAUNCHCHECK();
-  const dim3 grid((unsigned)nb, 27);
-#define SLBN_WGB(MT, NT) \
-  hipLaunchKernelGGL((k_wgrad_b<MT, NT>), grid, dim3(256), 0, nn_stream, Xb, Db, part, gpb, G)
-  const int mt = cout / 16, nt = cin / 16;
-  if (mt == 1 && nt == 1) SLBN_WGB(1, 1);
-  else if (mt == 1 && nt == 2) SLBN_WGB(1, 2);
-  else if (mt == 1 && nt == 4) SLBN_WGB(1, 4);
-  else if (mt == 2 && nt == 1) SLBN_WGB(2, 1);
-  else if (mt == 2 && nt == 2) SLBN_WGB(2, 2);
-  else if (mt == 2 && nt == 4) SLBN_WGB(2, 4);
-  else if (mt == 4 && nt == 1) SLBN_WGB(4, 1);
-  else if (mt == 4 && nt == 2) SLBN_WGB(4, 2);
-  else SLBN_WGB(4, 4);
+  const dim3 grid((unsigned)nb, 27 / tx);
+#define SLBN_WGB(MT, NT, TX)                                                             \
+  hipLaunchKernelGGL((k_wgrad_b<MT, NT, TX>), grid, dim3(256), 0, nn_stream, Xb, Db, part, \
+                     gpb, G)
+  if (mt == 1 && nt == 1) SLBN_WGB(1, 1, 3);
+  else if (mt == 1 && nt == 2) SLBN_WGB(1, 2, 3);
+  else if (mt == 2 && nt == 1) SLBN_WGB(2, 1, 3);
+  else if (mt == 1 && nt == 4) SLBN_WGB(1, 4, 1);
+  else if (mt == 2 && nt == 2) SLBN_WGB(2, 2, 1);
+  else if (mt == 2 && nt == 4) SLBN_WGB(2, 4, 1);
+  else if (mt == 4 && nt == 1) SLBN_WGB(4, 1, 1);
+  else if (mt == 4 && nt == 2) SLBN_WGB(4, 2, 1);
+  else SLBN_WGB(4, 4, 1);
 #undef SLBN_WGB
   LAUNCHCHECK();
   hipLaunchKernelGGL(k_wgrad_reduce, dim3(blocks_for(27L * cout * cin, 256)), dim3(256), 0,

@@ -140,6 +140,7 @@ def test_conv_input_gradient_matches_oracle(shape, cin, cout):
 @pytest.mark.parametrize("shape,cin,cout,small", [((16, 16, 16), 16, 16, 0), ((8, 16, 24), 32, 64, 0),
                                                   ((32, 32, 32), 64, 64, 0), ((16, 8, 8), 64, 32, 0),
                                                   ((8, 8, 12), 16, 32, 0), ((4, 6, 6), 64, 16, 0),
+                                                  ((8, 16, 16), 32, 16, 0),
                                                   ((8, 8, 12), 8, 16, 1), ((6, 8, 10), 16, 8, 1)])
 def test_conv_weight_gradient_matches_oracle(shape, cin, cout, small):
     L = lib()
