@@ -365,15 +365,31 @@ k_wgrad_b(const uint16_t *__restrict__ Xb, const uint16_t *__restrict__ Db,
 }
 
 /* dW[co][t*Cin + ci] += sum over blocks b (in order) of part[t][b][co][ci] */
-__global__ void k_wgrad_reduce(const float *__restrict__ part, int nb, int Cout, int Cin, int Kp,
-                               float *__restrict__ dW) {
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= 27 * Cout * Cin) return;
-  const int t = e / (Cout * Cin), rem = e - t * Cout * Cin, co = rem / Cin, ci = rem - co * Cin;
-  const float *p = part + (long)t * nb * Cout * Cin + rem;
-  float s = 0.f;
-  for (int b = 0; b < nb; b++) s += p[(long)b * Cout * Cin];
-  dW[(long)co * Kp + t * Cin + ci] += s;
+/* Block of 256 threads = 32 outputs x 8 strands: strand s sums the partials
+ * b = s, s + 8, ... in order, then the 8 strand sums are added in order
+ * (a fixed order: deterministic). */
+__global__ void __launch_bounds__(256)
+k_wgrad_reduce(const float *__restrict__ part, int nb, int Cout, int Cin, int Kp,
+               float *__restrict__ dW) {
+  __shared__ float red[8][32];
+  const int o = threadIdx.x & 31, sd = threadIdx.x >> 5;
+  const int e = blockIdx.x * 32 + o;
+  const bool ok = e < 27 * Cout * Cin;
+  float acc = 0.f;
+  if (ok) {
+    const int t = e / (Cout * Cin), rem = e - t * Cout * Cin;
+    const float *p = part + (long)t * nb * Cout * Cin + rem;
+    for (int b = sd; b < nb; b += 8) acc += p[(long)b * Cout * Cin];
+  }
+  red[sd][o] = acc;
+  __syncthreads();
+  if (sd == 0 && ok) {
+    float s = 0.f;
+#pragma unroll
+    for (int q = 0; q < 8; q++) s += red[q][o];
+    const int t = e / (Cout * Cin), rem = e - t * Cout * Cin, co = rem / Cin, ci = rem - co * Cin;
+    dW[(long)co * Kp + t * Cin + ci] += s;
+  }
 }
 
 /* ---------------------------------------------------------------------- */
@@ -669,7 +685,7 @@ extern "C" void slbn_conv_wgrad(const uint16_t *X, int cin, const uint16_t *dY, 
   else SLBN_WGB(4, 4, 1);
 #undef SLBN_WGB
   LAUNCHCHECK();
-  hipLaunchKernelGGL(k_wgrad_reduce, dim3(blocks_for(27L * cout * cin, 256)), dim3(256), 0,
+  hipLaunchKernelGGL(k_wgrad_reduce, dim3(blocks_for(27L * cout * cin, 32)), dim3(256), 0,
                      nn_stream, part, (int)nb, cout, cin, Kp, dW);
   LAUNCHCHECK();
 }
