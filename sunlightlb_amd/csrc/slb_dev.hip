@@ -1077,53 +1077,63 @@ struct VecSet {
   int m;
 };
 
+/* All population vectors hold a multiple of 4 values (15 pop_nodes(N), with
+ * pop_nodes a multiple of 1024) at 16-byte aligned addresses: the kernels
+ * below move 4 values per lane per access. */
 template <typename T>
-__global__ void k_vsub(T *__restrict__ z, const T *__restrict__ x, const T *__restrict__ y,
-                       long n) {
-  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+using V4 = T __attribute__((ext_vector_type(4)));
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_vsub(T *__restrict__ z, const T *__restrict__ x, const T *__restrict__ y, long n4) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n4;
        i += (long)gridDim.x * blockDim.x)
-    z[i] = x[i] - y[i];
+    ((V4<T> *)z)[i] = ((const V4<T> *)x)[i] - ((const V4<T> *)y)[i];
 }
 
 /* z1 = x1 - y1, z2 = x2 - y2 in one pass */
 template <typename T>
-__global__ void k_vsub2(T *__restrict__ z1, const T *__restrict__ x1, const T *__restrict__ y1,
-                        T *__restrict__ z2, const T *__restrict__ x2, const T *__restrict__ y2,
-                        long n) {
-  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+__global__ void __launch_bounds__(256)
+k_vsub2(T *__restrict__ z1, const T *__restrict__ x1, const T *__restrict__ y1,
+        T *__restrict__ z2, const T *__restrict__ x2, const T *__restrict__ y2, long n4) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n4;
        i += (long)gridDim.x * blockDim.x) {
-    z1[i] = x1[i] - y1[i];
-    z2[i] = x2[i] - y2[i];
+    ((V4<T> *)z1)[i] = ((const V4<T> *)x1)[i] - ((const V4<T> *)y1)[i];
+    ((V4<T> *)z2)[i] = ((const V4<T> *)x2)[i] - ((const V4<T> *)y2)[i];
   }
 }
 
-/* part[b * SLBD_VEC_MAX + j] = block b's share of a . B_j (FP64); with a2,
+/* part[b * SLBD_VEC_MAX + j] = block b's share of a . B_j (FP64); with TWO,
  * part[(gridDim + b) * SLBD_VEC_MAX + j] = block b's share of a2 . B_j, each
- * B_j read once for both */
-template <typename T, bool TWO>
+ * B_j read once for both.  M = number of vectors B_j (compile time). */
+template <typename T, int M, bool TWO>
 __global__ void __launch_bounds__(256)
-k_vdots(const T *__restrict__ a, const T *__restrict__ a2, VecSet B, long n,
+k_vdots(const T *__restrict__ a, const T *__restrict__ a2, VecSet B, long n4,
         double *__restrict__ part) {
   __shared__ double red[256];
-  double acc[SLBD_VEC_MAX], acc2[TWO ? SLBD_VEC_MAX : 1];
+  double acc[M], acc2[TWO ? M : 1];
 #pragma unroll
-  for (int j = 0; j < SLBD_VEC_MAX; j++) acc[j] = 0.0;
+  for (int j = 0; j < M; j++) acc[j] = 0.0;
 #pragma unroll
-  for (int j = 0; j < (TWO ? SLBD_VEC_MAX : 1); j++) acc2[j] = 0.0;
-  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+  for (int j = 0; j < (TWO ? M : 1); j++) acc2[j] = 0.0;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n4;
        i += (long)gridDim.x * blockDim.x) {
-    const double ai = (double)a[i];
-    const double bi = TWO ? (double)a2[i] : 0.0;
+    const V4<T> av = ((const V4<T> *)a)[i];
+    V4<T> bv = av;
+    if (TWO) bv = ((const V4<T> *)a2)[i];
 #pragma unroll
-    for (int j = 0; j < SLBD_VEC_MAX; j++)
-      if (j < B.m) {
-        const double v = (double)((const T *)B.p[j])[i];
-        acc[j] += ai * v;
-        if (TWO) acc2[j] += bi * v;
-      }
+    for (int j = 0; j < M; j++) {
+      const V4<T> v = ((const V4<T> *)B.p[j])[i];
+      acc[j] += (double)av.x * (double)v.x + (double)av.y * (double)v.y +
+                (double)av.z * (double)v.z + (double)av.w * (double)v.w;
+      if (TWO)
+        acc2[TWO ? j : 0] += (double)bv.x * (double)v.x + (double)bv.y * (double)v.y +
+                             (double)bv.z * (double)v.z + (double)bv.w * (double)v.w;
+    }
   }
   for (int q = 0; q < (TWO ? 2 : 1); q++)
-    for (int j = 0; j < B.m; j++) {
+#pragma unroll
+    for (int j = 0; j < M; j++) {
       red[threadIdx.x] = q ? acc2[TWO ? j : 0] : acc[j];
       __syncthreads();
       for (int s = blockDim.x / 2; s > 0; s >>= 1) {
@@ -1136,20 +1146,29 @@ k_vdots(const T *__restrict__ a, const T *__restrict__ a2, VecSet B, long n,
 }
 
 /* z = g - sum_j gamma_j B_j (z may be g) */
-template <typename T>
-__global__ void k_vmix(T *z, const T *g, VecSet B, long n) {
-  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+template <typename T, int M>
+__global__ void __launch_bounds__(256)
+k_vmix(T *z, const T *g, VecSet B, long n4) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n4;
        i += (long)gridDim.x * blockDim.x) {
-    double v = (double)g[i];
-    for (int j = 0; j < B.m; j++) v -= B.g[j] * (double)((const T *)B.p[j])[i];
-    z[i] = (T)v;
+    const V4<T> gv = ((const V4<T> *)g)[i];
+    double v0 = gv.x, v1 = gv.y, v2 = gv.z, v3 = gv.w;
+#pragma unroll
+    for (int j = 0; j < M; j++) {
+      const V4<T> b = ((const V4<T> *)B.p[j])[i];
+      const double c = B.g[j];
+      v0 -= c * (double)b.x; v1 -= c * (double)b.y; v2 -= c * (double)b.z; v3 -= c * (double)b.w;
+    }
+    V4<T> o;
+    o.x = (T)v0; o.y = (T)v1; o.z = (T)v2; o.w = (T)v3;
+    ((V4<T> *)z)[i] = o;
   }
 }
 
 static long vec_len(const slbd_lattice *L) { return 15 * pop_nodes(L->N); }
-static dim3 vec_grid(long n) {
-  const long b = (n + 255) / 256;
-  return dim3((unsigned)(b < RED_BLOCKS ? b : RED_BLOCKS));
+static dim3 vec_grid(long n4) {
+  const long b = (n4 + 255) / 256;
+  return dim3((unsigned)(b < 4096 ? b : 4096));
 }
 
 extern "C" void *slbd_vec_cur(slbd_lattice *L) { return L->pops[L->cur]; }
@@ -1172,14 +1191,37 @@ extern "C" void slbd_vec_copy(slbd_lattice *L, void *dst, const void *src) {
 }
 
 extern "C" void slbd_vec_sub(slbd_lattice *L, void *z, const void *x, const void *y) {
-  const long n = vec_len(L);
+  const long n4 = vec_len(L) / 4;
   if (L->prec == 32)
-    hipLaunchKernelGGL(k_vsub<float>, vec_grid(n), dim3(256), 0, L->stream, (float *)z,
-                       (const float *)x, (const float *)y, n);
+    hipLaunchKernelGGL(k_vsub<float>, vec_grid(n4), dim3(256), 0, L->stream, (float *)z,
+                       (const float *)x, (const float *)y, n4);
   else
-    hipLaunchKernelGGL(k_vsub<double>, vec_grid(n), dim3(256), 0, L->stream, (double *)z,
-                       (const double *)x, (const double *)y, n);
+    hipLaunchKernelGGL(k_vsub<double>, vec_grid(n4), dim3(256), 0, L->stream, (double *)z,
+                       (const double *)x, (const double *)y, n4);
   LAUNCHCHECK();
+}
+
+template <typename T, int M>
+static void vdots_launch(slbd_lattice *L, dim3 grid, const void *a, const void *a2,
+                         const VecSet &S, long n4) {
+  if (a2)
+    hipLaunchKernelGGL((k_vdots<T, M, true>), grid, dim3(256), 0, L->stream, (const T *)a,
+                       (const T *)a2, S, n4, L->part);
+  else
+    hipLaunchKernelGGL((k_vdots<T, M, false>), grid, dim3(256), 0, L->stream, (const T *)a,
+                       (const T *)nullptr, S, n4, L->part);
+}
+
+template <typename T>
+static void vdots_t(slbd_lattice *L, dim3 grid, const void *a, const void *a2, const VecSet &S,
+                    long n4) {
+  switch (S.m) {
+#define SLBD_VD(M) case M: vdots_launch<T, M>(L, grid, a, a2, S, n4); break;
+    SLBD_VD(1) SLBD_VD(2) SLBD_VD(3) SLBD_VD(4) SLBD_VD(5) SLBD_VD(6) SLBD_VD(7) SLBD_VD(8)
+    SLBD_VD(9) SLBD_VD(10) SLBD_VD(11) SLBD_VD(12) SLBD_VD(13) SLBD_VD(14) SLBD_VD(15)
+    SLBD_VD(16)
+#undef SLBD_VD
+  }
 }
 
 extern "C" void slbd_vec_dots2(slbd_lattice *L, const void *a, const void *a2, void *const *B,
@@ -1189,25 +1231,13 @@ extern "C" void slbd_vec_dots2(slbd_lattice *L, const void *a, const void *a2, v
   memset(&S, 0, sizeof S);
   S.m = m;
   for (int j = 0; j < m; j++) S.p[j] = B[j];
-  const long n = vec_len(L);
+  const long n4 = vec_len(L) / 4;
   /* two partial sets must fit L->part (RED_BLOCKS x 16 doubles) */
-  dim3 grid = vec_grid(n);
-  if (a2 && grid.x > RED_BLOCKS / 2) grid.x = RED_BLOCKS / 2;
-  if (L->prec == 32) {
-    if (a2)
-      hipLaunchKernelGGL((k_vdots<float, true>), grid, dim3(256), 0, L->stream,
-                         (const float *)a, (const float *)a2, S, n, L->part);
-    else
-      hipLaunchKernelGGL((k_vdots<float, false>), grid, dim3(256), 0, L->stream,
-                         (const float *)a, (const float *)nullptr, S, n, L->part);
-  } else {
-    if (a2)
-      hipLaunchKernelGGL((k_vdots<double, true>), grid, dim3(256), 0, L->stream,
-                         (const double *)a, (const double *)a2, S, n, L->part);
-    else
-      hipLaunchKernelGGL((k_vdots<double, false>), grid, dim3(256), 0, L->stream,
-                         (const double *)a, (const double *)nullptr, S, n, L->part);
-  }
+  dim3 grid((unsigned)((n4 + 255) / 256));
+  const unsigned cap = a2 ? RED_BLOCKS / 2 : RED_BLOCKS;
+  if (grid.x > cap) grid.x = cap;
+  if (L->prec == 32) vdots_t<float>(L, grid, a, a2, S, n4);
+  else vdots_t<double>(L, grid, a, a2, S, n4);
   LAUNCHCHECK();
   static double host[RED_BLOCKS * SLBD_VEC_MAX];
   const int sets = a2 ? 2 : 1;
@@ -1229,32 +1259,44 @@ extern "C" void slbd_vec_dots(slbd_lattice *L, const void *a, void *const *B, in
 
 extern "C" void slbd_vec_sub2(slbd_lattice *L, void *z1, const void *x1, const void *y1,
                               void *z2, const void *x2, const void *y2) {
-  const long n = vec_len(L);
+  const long n4 = vec_len(L) / 4;
   if (L->prec == 32)
-    hipLaunchKernelGGL(k_vsub2<float>, vec_grid(n), dim3(256), 0, L->stream, (float *)z1,
+    hipLaunchKernelGGL(k_vsub2<float>, vec_grid(n4), dim3(256), 0, L->stream, (float *)z1,
                        (const float *)x1, (const float *)y1, (float *)z2, (const float *)x2,
-                       (const float *)y2, n);
+                       (const float *)y2, n4);
   else
-    hipLaunchKernelGGL(k_vsub2<double>, vec_grid(n), dim3(256), 0, L->stream, (double *)z1,
+    hipLaunchKernelGGL(k_vsub2<double>, vec_grid(n4), dim3(256), 0, L->stream, (double *)z1,
                        (const double *)x1, (const double *)y1, (double *)z2,
-                       (const double *)x2, (const double *)y2, n);
+                       (const double *)x2, (const double *)y2, n4);
   LAUNCHCHECK();
+}
+
+template <typename T>
+static void vmix_t(slbd_lattice *L, void *z, const void *g, const VecSet &S, long n4) {
+  const dim3 grid = vec_grid(n4);
+  switch (S.m) {
+#define SLBD_VM(M)                                                                       \
+  case M:                                                                                \
+    hipLaunchKernelGGL((k_vmix<T, M>), grid, dim3(256), 0, L->stream, (T *)z, (const T *)g, \
+                       S, n4);                                                           \
+    break;
+    SLBD_VM(1) SLBD_VM(2) SLBD_VM(3) SLBD_VM(4) SLBD_VM(5) SLBD_VM(6) SLBD_VM(7) SLBD_VM(8)
+    SLBD_VM(9) SLBD_VM(10) SLBD_VM(11) SLBD_VM(12) SLBD_VM(13) SLBD_VM(14) SLBD_VM(15)
+    SLBD_VM(16)
+#undef SLBD_VM
+  }
 }
 
 extern "C" void slbd_vec_mix(slbd_lattice *L, void *z, const void *g, void *const *B,
                              const double *gamma, int m) {
-  if (m < 0 || m > SLBD_VEC_MAX) return;
+  if (m < 1 || m > SLBD_VEC_MAX) return;
   VecSet S;
   memset(&S, 0, sizeof S);
   S.m = m;
   for (int j = 0; j < m; j++) { S.p[j] = B[j]; S.g[j] = gamma[j]; }
-  const long n = vec_len(L);
-  if (L->prec == 32)
-    hipLaunchKernelGGL(k_vmix<float>, vec_grid(n), dim3(256), 0, L->stream, (float *)z,
-                       (const float *)g, S, n);
-  else
-    hipLaunchKernelGGL(k_vmix<double>, vec_grid(n), dim3(256), 0, L->stream, (double *)z,
-                       (const double *)g, S, n);
+  const long n4 = vec_len(L) / 4;
+  if (L->prec == 32) vmix_t<float>(L, z, g, S, n4);
+  else vmix_t<double>(L, z, g, S, n4);
   LAUNCHCHECK();
 }
 
