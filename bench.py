@@ -672,7 +672,7 @@ def main():
     ap.add_argument("--seed", type=int, default=20261202)
     ap.add_argument("--force", type=float, default=1e-6)
     ap.add_argument("--precision", type=int, default=32, choices=(32, 64))
-    ap.add_argument("--cpu-steps", type=int, default=4)
+    ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--cpu-mp-steps", type=int, default=100)
     ap.add_argument("--cpu-batch-steps", type=int, default=100)
     ap.add_argument("--no-cpu-baseline", action="store_true")
