@@ -39,6 +39,8 @@
 
 typedef __attribute__((ext_vector_type(8))) __bf16 bf16x8;
 typedef __attribute__((ext_vector_type(4))) float f32x4;
+typedef __attribute__((ext_vector_type(4))) uint32_t u32x4;
+__device__ __forceinline__ bf16x8 as_bf(u32x4 v) { return __builtin_bit_cast(bf16x8, v); }
 
 __device__ __forceinline__ float bf2f(uint16_t h) {
   return __uint_as_float(((uint32_t)h) << 16);
@@ -158,6 +160,98 @@ k_conv(const uint16_t *__restrict__ X, int Cin, const uint16_t *__restrict__ W, 
   }
 }
 
+/* The same convolution for 16 input channels (the full-resolution layers)
+ * with the input brick staged in LDS: a block computes 64 x-nodes x 4 rows
+ * of one plane (wave w: row w, four 16-node tiles) and first loads the
+ * 66 x 6 x 3 halo brick (38 KB, periodic wrap applied at load) once, so the
+ * 27 taps read the input from LDS instead of 27 times through L2.  The
+ * K loop, MFMA order and epilogue are those of k_conv: results are
+ * bit-identical.  Needs Wd % 64 == 0 and H % 4 == 0. */
+#define C16_TX 66
+#define C16_TY 6
+#define C16_TZ 3
+template <int MT>
+__global__ void __launch_bounds__(256)
+k_conv16_lds(const uint16_t *__restrict__ X, const uint16_t *__restrict__ W, int Kp,
+             const float *__restrict__ bias, uint16_t *__restrict__ Y, float *__restrict__ Yf,
+             int Cout, int relu, int sgn, NGeom G) {
+  __shared__ __attribute__((aligned(16))) uint16_t sX[C16_TZ * C16_TY * C16_TX * 16];
+  const int nbx = G.Wd >> 6, nby = G.H >> 2;
+  const int bx = blockIdx.x % nbx, rest = blockIdx.x / nbx;
+  const int by = rest % nby, z = rest / nby;
+  const int x0 = bx * 64, y0 = by * 4;
+  for (int e = threadIdx.x; e < C16_TZ * C16_TY * C16_TX * 2; e += blockDim.x) {
+    const int half = e & 1, node = e >> 1;
+    const int lx = node % C16_TX, r = node / C16_TX, ly = r % C16_TY, lz = r / C16_TY;
+    int gx = x0 + lx - 1, gy = y0 + ly - 1, gz = z + lz - 1;
+    gx += (gx < 0) ? G.Wd : 0; gx -= (gx >= G.Wd) ? G.Wd : 0;
+    gy += (gy < 0) ? G.H : 0; gy -= (gy >= G.H) ? G.H : 0;
+    gz += (gz < 0) ? G.D : 0; gz -= (gz >= G.D) ? G.D : 0;
+    const long src = ((long)(gz * G.H + gy) * G.Wd + gx) * 16 + half * 8;
+    *(u32x4 *)(sX + node * 16 + half * 8) = *(const u32x4 *)(X + src);
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, grp = lane >> 4, nl = lane & 15;
+  constexpr int K = 27 * 16;
+  f32x4 acc[4][MT];
+#pragma unroll
+  for (int u = 0; u < 4; u++)
+#pragma unroll
+    for (int m = 0; m < MT; m++) acc[u][m] = (f32x4){0.f, 0.f, 0.f, 0.f};
+  for (int k0 = 0; k0 < Kp; k0 += 32) {
+    const int kk = k0 + 8 * grp;
+    bf16x8 a[MT];
+#pragma unroll
+    for (int m = 0; m < MT; m++) {
+      const int co = 16 * m + nl;
+      a[m] = (bf16x8){0, 0, 0, 0, 0, 0, 0, 0};
+      if (co < Cout) a[m] = *(const bf16x8 *)(W + (long)co * Kp + kk);
+    }
+    bf16x8 b[4];
+    if (kk < K) {
+      const int t = kk >> 4, ci = kk & 15;
+      const int dz = (t / 9 - 1) * sgn, dy = ((t / 3) % 3 - 1) * sgn, dx = (t % 3 - 1) * sgn;
+      const int base = ((1 + dz) * C16_TY + (wv + 1 + dy)) * C16_TX + 1 + dx + nl;
+#pragma unroll
+      for (int u = 0; u < 4; u++)
+        b[u] = as_bf(*(const u32x4 *)(sX + (base + 16 * u) * 16 + ci));
+    } else {
+#pragma unroll
+      for (int u = 0; u < 4; u++) b[u] = (bf16x8){0, 0, 0, 0, 0, 0, 0, 0};
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+#pragma unroll
+      for (int m = 0; m < MT; m++)
+        acc[u][m] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[m], b[u], acc[u][m], 0, 0, 0);
+  }
+#pragma unroll
+  for (int u = 0; u < 4; u++) {
+    const int n = (z * G.H + y0 + wv) * G.Wd + x0 + 16 * u + nl;
+#pragma unroll
+    for (int m = 0; m < MT; m++) {
+      const int cb = 16 * m + 4 * grp;
+      if (cb >= Cout) continue;
+      float v[4];
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        v[r] = acc[u][m][r] + (bias ? bias[cb + r] : 0.f);
+        if (relu) v[r] = v[r] > 0.f ? v[r] : 0.f;
+      }
+      if (Y) {
+        uint2 pk;
+        pk.x = (uint32_t)f2bf(v[0]) | ((uint32_t)f2bf(v[1]) << 16);
+        pk.y = (uint32_t)f2bf(v[2]) | ((uint32_t)f2bf(v[3]) << 16);
+        *(uint2 *)(Y + (long)n * Cout + cb) = pk;
+      }
+      if (Yf) {
+#pragma unroll
+        for (int r = 0; r < 4; r++) Yf[(long)n * Cout + cb + r] = v[r];
+      }
+    }
+  }
+}
+
 /* ---------------------------------------------------------------------- */
 /* Weight gradient for one tap t: dW[co][t*Cin+ci] += sum_n dY[n][co] *     */
 /* X[n + off(t)][ci].  Block: 256 threads; wave w computes co tile w (if    */
@@ -249,9 +343,7 @@ __global__ void k_block(const uint16_t *__restrict__ X, int C, uint16_t *__restr
   *(uint4 *)(Xb + e * 8) = make_uint4(w[0], w[1], w[2], w[3]);
 }
 
-typedef __attribute__((ext_vector_type(4))) uint32_t u32x4;
 
-__device__ __forceinline__ bf16x8 as_bf(u32x4 v) { return __builtin_bit_cast(bf16x8, v); }
 
 template <int MT, int NT, int TX>
 __global__ void __launch_bounds__(256)
@@ -580,11 +672,27 @@ extern "C" void slbn_conv(const uint16_t *X, int cin, const uint16_t *W, const f
     fprintf(stderr, "slbn_conv: cin must be a multiple of 8, cout of 4, tensors < 2^31 elements\n");
     exit(EXIT_FAILURE);
   }
+  const int mt = (cout + 15) / 16;
+  /* 16 input channels on rows of 64k nodes: the LDS-staged kernel
+   * (SLBN_CONV_LDS=0 selects the general one) */
+  const char *lds_env = getenv("SLBN_CONV_LDS");
+  if (cin == 16 && Wd % 64 == 0 && H % 4 == 0 && mt <= 4 && !(lds_env && atoi(lds_env) == 0)) {
+    const dim3 g16((unsigned)((Wd / 64) * (H / 4) * D));
+#define SLBN_C16(MT)                                                                         \
+  hipLaunchKernelGGL((k_conv16_lds<MT>), g16, dim3(256), 0, nn_stream, X, W, Kp, bias, Y, Yf, \
+                     cout, relu, sgn, G)
+    if (mt == 1) SLBN_C16(1);
+    else if (mt == 2) SLBN_C16(2);
+    else if (mt == 3) SLBN_C16(3);
+    else SLBN_C16(4);
+#undef SLBN_C16
+    LAUNCHCHECK();
+    return;
+  }
 #define SLBN_CONV(MT)                                                                      \
   hipLaunchKernelGGL((k_conv<MT, SLBN_CONV_NTW>), dim3(g), dim3(256), 0, nn_stream, X, cin, W, \
                      Kp, bias, Y, Yf, \
                      cout, relu, sgn, G)
-  const int mt = (cout + 15) / 16;
   if (mt == 1) SLBN_CONV(1);
   else if (mt == 2) SLBN_CONV(2);
   else if (mt == 3) SLBN_CONV(3);

@@ -507,3 +507,39 @@ def test_start_policy_weights_file_and_cold_start():
         assert L.slb_warm_policy(net, 1e-6, 100) == 1
     finally:
         L.slb_warm_free(net)
+
+
+@pytest.mark.parametrize("shape,cout,transposed", [((4, 8, 64), 16, 0), ((3, 4, 128), 32, 1),
+                                                   ((4, 8, 64), 48, 0), ((2, 4, 64), 64, 1)])
+def test_conv16_lds_kernel_is_bit_identical_and_matches_oracle(shape, cout, transposed):
+    """16 input channels on rows of a multiple of 64 nodes take the LDS-staged
+    kernel; SLBN_CONV_LDS=0 forces the general one: outputs bit-identical
+    (same K order and MFMA sequence), and within FP32 rounding of the oracle."""
+    L = lib()
+    rng = np.random.default_rng(77 + cout + transposed)
+    N = int(np.prod(shape))
+    cin = 16
+    xb, x = bf_random(rng, shape + (cin,))
+    W, w = weights(rng, cout, cin)
+    bias = (rng.standard_normal(cout) * 0.1).astype(np.float32)
+    dX, dW, dB = Dev(host=xb), Dev(host=W), Dev(host=bias)
+    outs = []
+    for env in ("1", "0"):
+        os.environ["SLBN_CONV_LDS"] = env
+        try:
+            dY, dYf = Dev(N * cout * 2), Dev(N * cout * 4)
+            L.slbn_conv(dX.p, cin, dW.p, dB.p, dY.p, dYf.p, cout, 1, transposed, *shape)
+            L.slbn_sync()
+            outs.append((dY.get(np.uint16, shape + (cout,)), dYf.get(np.float32, shape + (cout,))))
+            dY.free()
+            dYf.free()
+        finally:
+            os.environ.pop("SLBN_CONV_LDS", None)
+    for d in (dX, dW, dB):
+        d.free()
+    assert np.array_equal(outs[0][0], outs[1][0])
+    assert np.array_equal(outs[0][1].view(np.uint32), outs[1][1].view(np.uint32))
+    if not transposed:
+        ref = np.maximum(onn.conv3d(x, w, bias.astype(np.float64)), 0.0)
+        err = np.abs(outs[0][1].astype(np.float64) - ref).max() / np.abs(ref).max()
+        assert err < 1e-5
