@@ -160,42 +160,42 @@ k_conv(const uint16_t *__restrict__ X, int Cin, const uint16_t *__restrict__ W, 
   }
 }
 
-/* The same convolution for 16 input channels (the full-resolution layers)
- * with the input brick staged in LDS: a block computes 64 x-nodes x 4 rows
- * of one plane (wave w: row w, four 16-node tiles) and first loads the
- * 66 x 6 x 3 halo brick (38 KB, periodic wrap applied at load) once, so the
- * 27 taps read the input from LDS instead of 27 times through L2.  The
- * K loop, MFMA order and epilogue are those of k_conv: results are
- * bit-identical.  Needs Wd % 64 == 0 and H % 4 == 0. */
-#define C16_TX 66
-#define C16_TY 6
-#define C16_TZ 3
-template <int MT>
+/* The same convolution for CIN = 16, 32 or 64 input channels with the input
+ * brick staged in LDS: a block computes XT = 1024/CIN x-nodes x 4 rows of
+ * one plane (wave w: row w, XT/16 tiles of 16 nodes) and first loads the
+ * (XT+2) x 6 x 3 halo brick (38-41 KB, periodic wrap applied at load) once,
+ * so the 27 taps read the input from LDS instead of 27 times through L2.
+ * The K loop, MFMA order and epilogue are those of k_conv: results are
+ * bit-identical.  Needs Wd % XT == 0 and H % 4 == 0. */
+#define CL_TY 6
+#define CL_TZ 3
+template <int CIN, int MT>
 __global__ void __launch_bounds__(256)
-k_conv16_lds(const uint16_t *__restrict__ X, const uint16_t *__restrict__ W, int Kp,
-             const float *__restrict__ bias, uint16_t *__restrict__ Y, float *__restrict__ Yf,
-             int Cout, int relu, int sgn, NGeom G) {
-  __shared__ __attribute__((aligned(16))) uint16_t sX[C16_TZ * C16_TY * C16_TX * 16];
-  const int nbx = G.Wd >> 6, nby = G.H >> 2;
+k_conv_lds(const uint16_t *__restrict__ X, const uint16_t *__restrict__ W, int Kp,
+           const float *__restrict__ bias, uint16_t *__restrict__ Y, float *__restrict__ Yf,
+           int Cout, int relu, int sgn, NGeom G) {
+  constexpr int XT = 1024 / CIN, NTW = XT / 16, TX = XT + 2, CH = CIN / 8;
+  __shared__ __attribute__((aligned(16))) uint16_t sX[CL_TZ * CL_TY * TX * CIN];
+  const int nbx = G.Wd / XT, nby = G.H >> 2;
   const int bx = blockIdx.x % nbx, rest = blockIdx.x / nbx;
   const int by = rest % nby, z = rest / nby;
-  const int x0 = bx * 64, y0 = by * 4;
-  for (int e = threadIdx.x; e < C16_TZ * C16_TY * C16_TX * 2; e += blockDim.x) {
-    const int half = e & 1, node = e >> 1;
-    const int lx = node % C16_TX, r = node / C16_TX, ly = r % C16_TY, lz = r / C16_TY;
+  const int x0 = bx * XT, y0 = by * 4;
+  for (int e = threadIdx.x; e < CL_TZ * CL_TY * TX * CH; e += blockDim.x) {
+    const int part = e % CH, node = e / CH;
+    const int lx = node % TX, r = node / TX, ly = r % CL_TY, lz = r / CL_TY;
     int gx = x0 + lx - 1, gy = y0 + ly - 1, gz = z + lz - 1;
     gx += (gx < 0) ? G.Wd : 0; gx -= (gx >= G.Wd) ? G.Wd : 0;
     gy += (gy < 0) ? G.H : 0; gy -= (gy >= G.H) ? G.H : 0;
     gz += (gz < 0) ? G.D : 0; gz -= (gz >= G.D) ? G.D : 0;
-    const long src = ((long)(gz * G.H + gy) * G.Wd + gx) * 16 + half * 8;
-    *(u32x4 *)(sX + node * 16 + half * 8) = *(const u32x4 *)(X + src);
+    const long src = ((long)(gz * G.H + gy) * G.Wd + gx) * CIN + part * 8;
+    *(u32x4 *)(sX + node * CIN + part * 8) = *(const u32x4 *)(X + src);
   }
   __syncthreads();
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, grp = lane >> 4, nl = lane & 15;
-  constexpr int K = 27 * 16;
-  f32x4 acc[4][MT];
+  constexpr int K = 27 * CIN;
+  f32x4 acc[NTW][MT];
 #pragma unroll
-  for (int u = 0; u < 4; u++)
+  for (int u = 0; u < NTW; u++)
 #pragma unroll
     for (int m = 0; m < MT; m++) acc[u][m] = (f32x4){0.f, 0.f, 0.f, 0.f};
   for (int k0 = 0; k0 < Kp; k0 += 32) {
@@ -207,26 +207,26 @@ k_conv16_lds(const uint16_t *__restrict__ X, const uint16_t *__restrict__ W, int
       a[m] = (bf16x8){0, 0, 0, 0, 0, 0, 0, 0};
       if (co < Cout) a[m] = *(const bf16x8 *)(W + (long)co * Kp + kk);
     }
-    bf16x8 b[4];
+    bf16x8 b[NTW];
     if (kk < K) {
-      const int t = kk >> 4, ci = kk & 15;
+      const int t = kk / CIN, ci = kk % CIN;
       const int dz = (t / 9 - 1) * sgn, dy = ((t / 3) % 3 - 1) * sgn, dx = (t % 3 - 1) * sgn;
-      const int base = ((1 + dz) * C16_TY + (wv + 1 + dy)) * C16_TX + 1 + dx + nl;
+      const int base = ((1 + dz) * CL_TY + (wv + 1 + dy)) * TX + 1 + dx + nl;
 #pragma unroll
-      for (int u = 0; u < 4; u++)
-        b[u] = as_bf(*(const u32x4 *)(sX + (base + 16 * u) * 16 + ci));
+      for (int u = 0; u < NTW; u++)
+        b[u] = as_bf(*(const u32x4 *)(sX + (base + 16 * u) * CIN + ci));
     } else {
 #pragma unroll
-      for (int u = 0; u < 4; u++) b[u] = (bf16x8){0, 0, 0, 0, 0, 0, 0, 0};
+      for (int u = 0; u < NTW; u++) b[u] = (bf16x8){0, 0, 0, 0, 0, 0, 0, 0};
     }
 #pragma unroll
-    for (int u = 0; u < 4; u++)
+    for (int u = 0; u < NTW; u++)
 #pragma unroll
       for (int m = 0; m < MT; m++)
         acc[u][m] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[m], b[u], acc[u][m], 0, 0, 0);
   }
 #pragma unroll
-  for (int u = 0; u < 4; u++) {
+  for (int u = 0; u < NTW; u++) {
     const int n = (z * G.H + y0 + wv) * G.Wd + x0 + 16 * u + nl;
 #pragma unroll
     for (int m = 0; m < MT; m++) {
@@ -673,19 +673,24 @@ extern "C" void slbn_conv(const uint16_t *X, int cin, const uint16_t *W, const f
     exit(EXIT_FAILURE);
   }
   const int mt = (cout + 15) / 16;
-  /* 16 input channels on rows of 64k nodes: the LDS-staged kernel
-   * (SLBN_CONV_LDS=0 selects the general one) */
+  /* 16, 32 or 64 input channels on rows of a multiple of 1024/cin nodes:
+   * the LDS-staged kernel (SLBN_CONV_LDS=0 selects the general one) */
   const char *lds_env = getenv("SLBN_CONV_LDS");
-  if (cin == 16 && Wd % 64 == 0 && H % 4 == 0 && mt <= 4 && !(lds_env && atoi(lds_env) == 0)) {
-    const dim3 g16((unsigned)((Wd / 64) * (H / 4) * D));
-#define SLBN_C16(MT)                                                                         \
-  hipLaunchKernelGGL((k_conv16_lds<MT>), g16, dim3(256), 0, nn_stream, X, W, Kp, bias, Y, Yf, \
+  const int xt = cin >= 16 ? 1024 / cin : 0;
+  if ((cin == 16 || cin == 32 || cin == 64) && Wd % xt == 0 && H % 4 == 0 && mt <= 4 &&
+      !(lds_env && atoi(lds_env) == 0)) {
+    const dim3 gl((unsigned)((Wd / xt) * (H / 4) * D));
+#define SLBN_CL(C, MT)                                                                      \
+  hipLaunchKernelGGL((k_conv_lds<C, MT>), gl, dim3(256), 0, nn_stream, X, W, Kp, bias, Y, Yf, \
                      cout, relu, sgn, G)
-    if (mt == 1) SLBN_C16(1);
-    else if (mt == 2) SLBN_C16(2);
-    else if (mt == 3) SLBN_C16(3);
-    else SLBN_C16(4);
-#undef SLBN_C16
+#define SLBN_CLM(C)                                                                         \
+  if (mt == 1) SLBN_CL(C, 1); else if (mt == 2) SLBN_CL(C, 2);                              \
+  else if (mt == 3) SLBN_CL(C, 3); else SLBN_CL(C, 4);
+    if (cin == 16) { SLBN_CLM(16) }
+    else if (cin == 32) { SLBN_CLM(32) }
+    else { SLBN_CLM(64) }
+#undef SLBN_CLM
+#undef SLBN_CL
     LAUNCHCHECK();
     return;
   }

@@ -509,16 +509,18 @@ def test_start_policy_weights_file_and_cold_start():
         L.slb_warm_free(net)
 
 
-@pytest.mark.parametrize("shape,cout,transposed", [((4, 8, 64), 16, 0), ((3, 4, 128), 32, 1),
-                                                   ((4, 8, 64), 48, 0), ((2, 4, 64), 64, 1)])
-def test_conv16_lds_kernel_is_bit_identical_and_matches_oracle(shape, cout, transposed):
-    """16 input channels on rows of a multiple of 64 nodes take the LDS-staged
-    kernel; SLBN_CONV_LDS=0 forces the general one: outputs bit-identical
-    (same K order and MFMA sequence), and within FP32 rounding of the oracle."""
+@pytest.mark.parametrize("shape,cin,cout,transposed", [
+    ((4, 8, 64), 16, 16, 0), ((3, 4, 128), 16, 32, 1), ((4, 8, 64), 16, 48, 0),
+    ((2, 4, 64), 16, 64, 1), ((4, 8, 32), 32, 32, 0), ((3, 4, 64), 32, 64, 1),
+    ((4, 4, 16), 64, 64, 0), ((2, 8, 32), 64, 16, 1)])
+def test_conv_lds_kernel_is_bit_identical_and_matches_oracle(shape, cin, cout, transposed):
+    """16, 32 or 64 input channels on rows of a multiple of 1024/cin nodes take
+    the LDS-staged kernel; SLBN_CONV_LDS=0 forces the general one: outputs
+    bit-identical (same K order and MFMA sequence), and within FP32 rounding
+    of the oracle."""
     L = lib()
-    rng = np.random.default_rng(77 + cout + transposed)
+    rng = np.random.default_rng(77 + cin + cout + transposed)
     N = int(np.prod(shape))
-    cin = 16
     xb, x = bf_random(rng, shape + (cin,))
     W, w = weights(rng, cout, cin)
     bias = (rng.standard_normal(cout) * 0.1).astype(np.float32)
