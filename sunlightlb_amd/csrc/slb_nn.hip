@@ -638,6 +638,114 @@ __global__ void k_bf2f(const uint16_t *__restrict__ a, float *__restrict__ b, lo
   if (e < n) b[e] = bf2f(a[e]);
 }
 
+/* 16-byte forms of the element-wise kernels: 8 BF16 values per thread, for
+ * tensors whose element count (or channel count) is a multiple of 8.  Each
+ * value is computed exactly as by the scalar kernel above. */
+__device__ __forceinline__ void unpack8(u32x4 v, float f[8]) {
+  const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    f[2 * q] = __uint_as_float(w[q] << 16);
+    f[2 * q + 1] = __uint_as_float(w[q] & 0xffff0000u);
+  }
+}
+__device__ __forceinline__ u32x4 pack8(const float f[8]) {
+  uint32_t w[4];
+#pragma unroll
+  for (int q = 0; q < 4; q++) w[q] = (uint32_t)f2bf(f[2 * q]) | ((uint32_t)f2bf(f[2 * q + 1]) << 16);
+  return (u32x4){w[0], w[1], w[2], w[3]};
+}
+
+__global__ void k_relu_bwd8(const uint16_t *__restrict__ Y, const uint16_t *__restrict__ dY,
+                            uint16_t *__restrict__ dP, long n8) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n8) return;
+  const u32x4 y = ((const u32x4 *)Y)[e], d = ((const u32x4 *)dY)[e];
+  const uint32_t yw[4] = {y.x, y.y, y.z, y.w}, dw[4] = {d.x, d.y, d.z, d.w};
+  uint32_t o[4];
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    const bool lo = __uint_as_float(yw[q] << 16) > 0.f;
+    const bool hi = __uint_as_float(yw[q] & 0xffff0000u) > 0.f;
+    o[q] = (lo ? (dw[q] & 0xffffu) : 0u) | (hi ? (dw[q] & 0xffff0000u) : 0u);
+  }
+  ((u32x4 *)dP)[e] = (u32x4){o[0], o[1], o[2], o[3]};
+}
+
+__global__ void k_add_into8(uint16_t *__restrict__ A, const uint16_t *__restrict__ B, long n8) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n8) return;
+  float a[8], b[8];
+  unpack8(((const u32x4 *)A)[e], a);
+  unpack8(((const u32x4 *)B)[e], b);
+#pragma unroll
+  for (int q = 0; q < 8; q++) a[q] += b[q];
+  ((u32x4 *)A)[e] = pack8(a);
+}
+
+/* element e8 = (node, channel group of 8) */
+__global__ void k_up_add8(const uint16_t *__restrict__ Xc, const uint16_t *__restrict__ S,
+                          uint16_t *__restrict__ Y, int C, NGeom Gf, NGeom Gc) {
+  const int G8 = C >> 3;
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long)Gf.N * G8) return;
+  const int n = (int)(e / G8), g = (int)(e - (long)n * G8);
+  const int z = n / Gf.plane, rem = n - z * Gf.plane, y = rem / Gf.Wd, x = rem - y * Gf.Wd;
+  const int m = ((z / 2) * Gc.H + y / 2) * Gc.Wd + x / 2;
+  float a[8], b[8];
+  unpack8(*(const u32x4 *)(Xc + (long)m * C + 8 * g), a);
+  if (S) {
+    unpack8(((const u32x4 *)S)[e], b);
+#pragma unroll
+    for (int q = 0; q < 8; q++) a[q] += b[q];
+  } else {
+#pragma unroll
+    for (int q = 0; q < 8; q++) a[q] += 0.f;
+  }
+  ((u32x4 *)Y)[e] = pack8(a);
+}
+
+__global__ void k_pool_bwd8(const uint16_t *__restrict__ dY, uint16_t *__restrict__ dX, int C,
+                            NGeom Gi, NGeom Go) {
+  const int G8 = C >> 3;
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long)Gi.N * G8) return;
+  const int n = (int)(e / G8), g = (int)(e - (long)n * G8);
+  const int z = n / Gi.plane, rem = n - z * Gi.plane, y = rem / Gi.Wd, x = rem - y * Gi.Wd;
+  const int m = ((z / 2) * Go.H + y / 2) * Go.Wd + x / 2;
+  float a[8];
+  unpack8(*(const u32x4 *)(dY + (long)m * C + 8 * g), a);
+#pragma unroll
+  for (int q = 0; q < 8; q++) a[q] = 0.125f * a[q];
+  ((u32x4 *)dX)[e] = pack8(a);
+}
+
+/* db[c] += sum_n dP[n][c]: thread t sums channel group t % (C/8) over rows
+ * t / (C/8), + 256/(C/8), ... of its block's grid-stride share, the block
+ * reduces in LDS in a fixed order and adds one value per channel */
+__global__ void __launch_bounds__(256)
+k_bias_grad8(const uint16_t *__restrict__ dP, int C, int N, float *__restrict__ db) {
+  __shared__ float sm[256 * 8];
+  const int G8 = C >> 3, R = 256 / G8;
+  const int g = threadIdx.x % G8, r = threadIdx.x / G8;
+  float acc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  for (long n = (long)blockIdx.x * R + r; n < N; n += (long)gridDim.x * R) {
+    float f[8];
+    unpack8(*(const u32x4 *)(dP + n * C + 8 * g), f);
+#pragma unroll
+    for (int q = 0; q < 8; q++) acc[q] += f[q];
+  }
+#pragma unroll
+  for (int q = 0; q < 8; q++) sm[threadIdx.x * 8 + q] = acc[q];
+  __syncthreads();
+  if (threadIdx.x < C) {
+    const int gg = threadIdx.x >> 3, q = threadIdx.x & 7;
+    float t = 0.f;
+    for (int k = 0; k < R; k++) t += sm[(k * G8 + gg) * 8 + q];
+    atomicAdd(db + threadIdx.x, t);
+  }
+}
+
 /* ---------------------------------------------------------------------- */
 /* Host ABI.                                                                */
 
@@ -811,7 +919,12 @@ extern "C" void slbn_wtrans(const uint16_t *W, int cout, int cin, uint16_t *WT) 
 }
 
 extern "C" void slbn_relu_bwd(const uint16_t *Y, const uint16_t *dY, uint16_t *dP, long n) {
-  hipLaunchKernelGGL(k_relu_bwd, dim3(blocks_for(n, 256)), dim3(256), 0, nn_stream, Y, dY, dP, n);
+  if (n % 8 == 0)
+    hipLaunchKernelGGL(k_relu_bwd8, dim3(blocks_for(n / 8, 256)), dim3(256), 0, nn_stream, Y, dY,
+                       dP, n / 8);
+  else
+    hipLaunchKernelGGL(k_relu_bwd, dim3(blocks_for(n, 256)), dim3(256), 0, nn_stream, Y, dY, dP,
+                       n);
   LAUNCHCHECK();
 }
 
@@ -819,6 +932,15 @@ extern "C" void slbn_bias_grad(const uint16_t *dP, int C, int N, float *db) {
   if (C < 1 || C > 256 || 256 % C) {
     fprintf(stderr, "slbn_bias_grad: the channel count must divide 256\n");
     exit(EXIT_FAILURE);
+  }
+  if (C % 8 == 0) {
+    const long rows = 256 / (C / 8);
+    long nb = ((long)N + rows - 1) / rows;
+    if (nb > 1024) nb = 1024;
+    if (nb < 1) nb = 1;
+    hipLaunchKernelGGL(k_bias_grad8, dim3((unsigned)nb), dim3(256), 0, nn_stream, dP, C, N, db);
+    LAUNCHCHECK();
+    return;
   }
   const long rows = 256 / C;
   long nb = ((long)N + rows - 1) / rows;
@@ -836,15 +958,23 @@ extern "C" void slbn_pool(const uint16_t *X, uint16_t *Y, int C, int D, int H, i
 }
 extern "C" void slbn_pool_bwd(const uint16_t *dY, uint16_t *dX, int C, int D, int H, int Wd) {
   const NGeom Gi = ngeom(D, H, Wd), Go = ngeom(D / 2, H / 2, Wd / 2);
-  hipLaunchKernelGGL(k_pool_bwd, dim3(blocks_for((long)Gi.N * C, 256)), dim3(256), 0, nn_stream,
-                     dY, dX, C, Gi, Go);
+  if (C % 8 == 0)
+    hipLaunchKernelGGL(k_pool_bwd8, dim3(blocks_for((long)Gi.N * (C / 8), 256)), dim3(256), 0,
+                       nn_stream, dY, dX, C, Gi, Go);
+  else
+    hipLaunchKernelGGL(k_pool_bwd, dim3(blocks_for((long)Gi.N * C, 256)), dim3(256), 0,
+                       nn_stream, dY, dX, C, Gi, Go);
   LAUNCHCHECK();
 }
 extern "C" void slbn_up_add(const uint16_t *Xc, const uint16_t *S, uint16_t *Y, int C, int D, int H,
                             int Wd) {
   const NGeom Gf = ngeom(D, H, Wd), Gc = ngeom(D / 2, H / 2, Wd / 2);
-  hipLaunchKernelGGL(k_up_add, dim3(blocks_for((long)Gf.N * C, 256)), dim3(256), 0, nn_stream, Xc,
-                     S, Y, C, Gf, Gc);
+  if (C % 8 == 0)
+    hipLaunchKernelGGL(k_up_add8, dim3(blocks_for((long)Gf.N * (C / 8), 256)), dim3(256), 0,
+                       nn_stream, Xc, S, Y, C, Gf, Gc);
+  else
+    hipLaunchKernelGGL(k_up_add, dim3(blocks_for((long)Gf.N * C, 256)), dim3(256), 0, nn_stream,
+                       Xc, S, Y, C, Gf, Gc);
   LAUNCHCHECK();
 }
 extern "C" void slbn_up_bwd(const uint16_t *dY, uint16_t *dXc, int C, int D, int H, int Wd) {
@@ -854,7 +984,11 @@ extern "C" void slbn_up_bwd(const uint16_t *dY, uint16_t *dXc, int C, int D, int
   LAUNCHCHECK();
 }
 extern "C" void slbn_add_into(uint16_t *A, const uint16_t *B, long n) {
-  hipLaunchKernelGGL(k_add_into, dim3(blocks_for(n, 256)), dim3(256), 0, nn_stream, A, B, n);
+  if (n % 8 == 0)
+    hipLaunchKernelGGL(k_add_into8, dim3(blocks_for(n / 8, 256)), dim3(256), 0, nn_stream, A, B,
+                       n / 8);
+  else
+    hipLaunchKernelGGL(k_add_into, dim3(blocks_for(n, 256)), dim3(256), 0, nn_stream, A, B, n);
   LAUNCHCHECK();
 }
 extern "C" void slbn_mse(const float *Yf, int C, const uint16_t *T, int Ct, int Cv, int N,
