@@ -612,6 +612,56 @@ k_mse(const float *__restrict__ Yf, int C, const uint16_t *__restrict__ T, int C
   if (threadIdx.x == 0) atomicAdd(loss, sm[0] + sm[1] + sm[2] + sm[3]);
 }
 
+/* The same loss for C = 16 prediction channels and an even Ct <= 16: one
+ * node per thread, its 16 FP32 predictions read as four 16-byte vectors, the
+ * Ct BF16 targets as Ct/2 words and its 16 gradients written as two 16-byte
+ * vectors (every value computed as by k_mse). */
+__global__ void __launch_bounds__(256)
+k_mse16(const float *__restrict__ Yf, const uint16_t *__restrict__ T, int Ct, int Cv, int N,
+        const float *__restrict__ wc, uint16_t *__restrict__ dY, float *__restrict__ loss) {
+  __shared__ float sm[4];
+  float w[16];
+#pragma unroll
+  for (int c = 0; c < 16; c++) w[c] = c < Cv ? wc[c] : 0.f;
+  float s = 0.f;
+  for (long n = (long)blockIdx.x * blockDim.x + threadIdx.x; n < N;
+       n += (long)gridDim.x * blockDim.x) {
+    float y[16], t[16];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const f32x4 v = ((const f32x4 *)(Yf + n * 16))[q];
+      y[4 * q] = v[0]; y[4 * q + 1] = v[1]; y[4 * q + 2] = v[2]; y[4 * q + 3] = v[3];
+    }
+    const uint32_t *tw = (const uint32_t *)(T + n * Ct);
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+      const uint32_t u = (2 * q < Ct) ? tw[q] : 0u;
+      t[2 * q] = __uint_as_float(u << 16);
+      t[2 * q + 1] = __uint_as_float(u & 0xffff0000u);
+    }
+    float g[16];
+#pragma unroll
+    for (int c = 0; c < 16; c++) {
+      g[c] = 0.f;
+      if (c < Cv) {
+        const float d = y[c] - t[c];
+        s += w[c] * d * d;
+        g[c] = 2.f * d * w[c];
+      }
+    }
+    uint32_t o[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++) o[q] = (uint32_t)f2bf(g[2 * q]) | ((uint32_t)f2bf(g[2 * q + 1]) << 16);
+    u32x4 *dst = (u32x4 *)(dY + n * 16);
+    dst[0] = (u32x4){o[0], o[1], o[2], o[3]};
+    dst[1] = (u32x4){o[4], o[5], o[6], o[7]};
+  }
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) atomicAdd(loss, sm[0] + sm[1] + sm[2] + sm[3]);
+}
+
 /* Adam on FP32 master weights, refreshing the BF16 copy (zero-padded K
  * columns stay zero because their gradient is zero). */
 __global__ void k_adam(float *__restrict__ w, uint16_t *__restrict__ wb, const float *__restrict__ g,
@@ -995,7 +1045,12 @@ extern "C" void slbn_mse(const float *Yf, int C, const uint16_t *T, int Ct, int 
                          const float *wc, uint16_t *dY, float *loss) {
   unsigned nb = blocks_for(N, 256);
   if (nb > 2048) nb = 2048;
-  hipLaunchKernelGGL(k_mse, dim3(nb), dim3(256), 0, nn_stream, Yf, C, T, Ct, Cv, N, wc, dY, loss);
+  if (C == 16 && Ct % 2 == 0 && Ct <= 16 && Cv <= Ct)
+    hipLaunchKernelGGL(k_mse16, dim3(nb), dim3(256), 0, nn_stream, Yf, T, Ct, Cv, N, wc, dY,
+                       loss);
+  else
+    hipLaunchKernelGGL(k_mse, dim3(nb), dim3(256), 0, nn_stream, Yf, C, T, Ct, Cv, N, wc, dY,
+                       loss);
   LAUNCHCHECK();
 }
 extern "C" void slbn_adam(float *w, uint16_t *wb, const float *g, float *m, float *v, long n,
