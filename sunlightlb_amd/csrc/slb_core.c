@@ -8,6 +8,7 @@
  * makemask, syncobjs, countfluid, countmask, vspread, zerobf, addbf, vset
  * (src/lbobjs.c:526-714).
  */
+#include <limits.h>
 #include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -43,6 +44,10 @@ static int default_precision(void) {
 }
 
 LBSys newlb(int reqesi, int reqesj, int reqesk, char *name) {
+  /* nnodes is an int in the struct layout drivers link against */
+  if (reqesi < 1 || reqesj < 1 || reqesk < 1 ||
+      (long)reqesi * reqesj * reqesk > (long)INT_MAX)
+    error("newlb: lattice dimensions must be positive, with at most 2^31 - 1 nodes");
   slb_lbwrap *W = (slb_lbwrap *)calloc(1, sizeof(slb_lbwrap));
   if (W == NULL) error("No room for lbsys structure\n");
   LBSys lb = &W->pub;

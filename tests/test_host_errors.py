@@ -29,3 +29,17 @@ def test_bench_refuses_several_gpus():
                        capture_output=True, text=True, timeout=120)
     assert p.returncode != 0
     assert "one GPU" in (p.stdout + p.stderr)
+
+
+def test_newlb_rejects_bad_dimensions():
+    """Non-positive dimensions, or more nodes than the int node count holds,
+    end the process with the library's error message before any device work
+    (the check precedes device set-up, so it runs without a GPU)."""
+    for dims in ("0, 4, 4", "4, -1, 4", "2048, 1024, 1024"):
+        code = ("import sunlightlb_amd as s; L = s.lib(); "
+                f"L.newlb({dims}, b'bad'); print('returned')")
+        p = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True,
+                           timeout=120, cwd=ROOT)
+        assert p.returncode == 1, (dims, p.stdout, p.stderr)
+        assert "lattice dimensions" in p.stderr
+        assert "returned" not in p.stdout
