@@ -212,16 +212,9 @@ def converge(L, args):
     return res
 
 
-def mp_workload(L, args, n=128, R=6.0, phi=0.25, peclet=10.0, alpha=0.0, pads=0.01):
-    """Passive-scalar moment propagation on the steady flow of a sphere pack.
-
-    Steady Stokes flow of the pack (cold start to the residual tolerance), the
-    populations scaled so that Pe = U (2R) / D = peclet with U the
-    interstitial mean speed (superficial vmean / porosity) and D = (1 -
-    alpha)/6; uniform unit tracer on fluid nodes, partial adsorption pads at
-    boundary solid nodes, displacement moments along x, mp_steps steps in the
-    lattice precision with FP64 moment sums."""
-    os.environ["SLB_PRECISION"] = str(args.precision)
+def mp_run(L, args, prec, n, R, phi, peclet, alpha, pads, baseline):
+    """One MP workload run in the given lattice precision (see mp_workload)."""
+    os.environ["SLB_PRECISION"] = str(prec)
     h, nsph, phi_got = build_pack(L, n, R, phi, args.seed + 1, 1e-6)
     r = C.c_double(0.0)
     steps = L.slb_run_to_residual(h, args.tol, args.interval, args.maxsteps, C.byref(r))
@@ -253,7 +246,7 @@ def mp_workload(L, args, n=128, R=6.0, phi=0.25, peclet=10.0, alpha=0.0, pads=0.
     N = n ** 3
     mlups = N * args.mp_steps / (ms * 1e-3) / 1e6
     # 15 populations + one 4-value node record {P, m1, m2, 1/rho} read and written
-    bpl = (15 + 2 * 4) * (4 if args.precision == 32 else 8)
+    bpl = (15 + 2 * 4) * (4 if prec == 32 else 8)
     res = {
         "workload": f"{n}^3 RSA pack R {R:g} phi {phi_got:.4f} ({nsph} spheres), flow solved "
                     f"to residual {r.value:.2e} in {steps} steps, scaled to Pe {peclet:g} "
@@ -269,10 +262,35 @@ def mp_workload(L, args, n=128, R=6.0, phi=0.25, peclet=10.0, alpha=0.0, pads=0.
         "algorithmic_bytes_per_node": bpl,
         "achieved_GBs": round(bpl * N / (ms / args.mp_steps * 1e-3) / 1e9, 1),
     }
-    if not args.no_cpu_baseline:
+    if baseline:
         res["cpu_baseline"] = cpu_mp_baseline(L, h, n, args.cpu_mp_steps)
-    log(f"mp: {res}")
     L.slb_destroy(h)
+    return res
+
+
+MP_QUANTITIES = ("surviving_fraction", "adsorbed_fraction", "mean_displacement_x",
+                 "variance_displacement_x")
+
+
+def mp_workload(L, args, n=128, R=6.0, phi=0.25, peclet=10.0, alpha=0.0, pads=0.01):
+    """Passive-scalar moment propagation on the steady flow of a sphere pack.
+
+    Steady Stokes flow of the pack (cold start to the residual tolerance), the
+    populations scaled so that Pe = U (2R) / D = peclet with U the
+    interstitial mean speed (superficial vmean / porosity) and D = (1 -
+    alpha)/6; uniform unit tracer on fluid nodes, partial adsorption pads at
+    boundary solid nodes, displacement moments along x, mp_steps steps in the
+    lattice precision with FP64 moment sums.  An FP32 run is repeated in FP64
+    (flow solve included) and every reported quantity carries its relative
+    difference from the FP64 run."""
+    res = mp_run(L, args, args.precision, n, R, phi, peclet, alpha, pads,
+                 not args.no_cpu_baseline)
+    if args.precision == 32:
+        r64 = mp_run(L, args, 64, n, R, phi, peclet, alpha, pads, False)
+        res["fp64"] = {q: r64[q] for q in MP_QUANTITIES + ("ms_per_step",)}
+        res["rel_diff_fp32_vs_fp64"] = {
+            q: abs(res[q] - r64[q]) / abs(r64[q]) for q in MP_QUANTITIES}
+    log(f"mp: {res}")
     return res
 
 
@@ -321,17 +339,12 @@ def insert_periodic(L, h, Lb, R, c):
     return objs
 
 
-def batch_workload(L, args, u0=1e-3, tau=1.0):
-    """512 independent periodic boxes, each with 6 right-hand sides (unit
-    translation and rotation of sphere 0, the other spheres fixed), stepped
-    together: one launch per step over all problems, autotuned launch shape,
-    HIP-graph replay.  Reports throughput and the 6x6 resistance matrices of
-    sphere 0 after measure()."""
+def batch_run(L, args, prec, boxes, u0, tau, tune):
+    """One batched run in the given precision: returns the throughput record
+    and the 6x6 resistance matrix of sphere 0 in every box."""
     import numpy as np
-    prec = args.precision
     os.environ["SLB_PRECISION"] = str(prec)
     t0 = time.perf_counter()
-    boxes = make_boxes(args.batch_boxes, args.seed + 7)
     b = L.slb_batch_new(prec)
     moving_rows = []
     for (Lb, sph) in boxes:
@@ -361,8 +374,9 @@ def batch_workload(L, args, u0=1e-3, tau=1.0):
     L.slb_batch_commit(b)
     setup = time.perf_counter() - t0
     t1 = time.perf_counter()
-    L.slb_batch_autotune(b, None)
-    tune = time.perf_counter() - t1
+    if tune:
+        L.slb_batch_autotune(b, None)
+    tune_s = time.perf_counter() - t1
     cfg = (C.c_int * 4)()
     L.slb_batch_get_config(b, cfg)
     nodes = L.slb_batch_nodes(b)
@@ -371,8 +385,7 @@ def batch_workload(L, args, u0=1e-3, tau=1.0):
     L.slb_batch_measure(b)
     mlups = nodes * args.batch_steps / (ms * 1e-3) / 1e6
     bpl = 2 * 15 * (4 if prec == 32 else 8)
-    eta = 72.0 * (tau - 0.5) / 3.0
-    drag, asym = [], []
+    mats = []
     for q, (rows, nobj, R0) in enumerate(moving_rows):
         Rm = np.zeros((6, 6))
         for k in range(6):
@@ -384,25 +397,47 @@ def batch_workload(L, args, u0=1e-3, tau=1.0):
             T = trq.reshape(nobj, 3)[rows].sum(0)
             amp = u0 if k < 3 else u0 / R0
             Rm[:, k] = np.concatenate([F, T]) / amp
-        drag.append(-np.mean(np.diag(Rm)[:3]) / (6 * np.pi * eta * R0))
-        asym.append(np.abs(Rm - Rm.T).max() / np.abs(Rm).max())
+        mats.append(Rm)
     L.slb_batch_free(b)
     res = {
-        "workload": f"{args.batch_boxes} periodic boxes, edge U{{32..64}}, 2-20 spheres radius "
+        "workload": f"{len(boxes)} periodic boxes, edge U{{32..64}}, 2-20 spheres radius "
                     f"U[3,6] (seed {args.seed + 7}), sphere 0 translating/rotating (6 rhs), "
                     f"Stokes, tau {tau:g}, {'f32' if prec == 32 else 'f64'}",
-        "problems": 6 * args.batch_boxes,
+        "problems": 6 * len(boxes),
         "nodes_per_step": nodes,
         "steps": args.batch_steps,
         "ms_per_step": round(ms / args.batch_steps, 4),
         "MLUPS": round(mlups, 1),
         "achieved_GBs": round(bpl * nodes / (ms / args.batch_steps * 1e-3) / 1e9, 1),
         "launch": {"block": cfg[0], "nodes_per_thread": cfg[1], "grid": cfg[2],
-                   "tiles": cfg[3], "autotune_s": round(tune, 2)},
+                   "tiles": cfg[3], "autotune_s": round(tune_s, 2)},
         "setup_s": round(setup, 1),
-        "translational_drag_over_stokes_median": float(np.median(drag)),
-        "resistance_asymmetry_median": float(np.median(asym)),
     }
+    return res, mats
+
+
+def batch_workload(L, args, u0=1e-3, tau=1.0):
+    """512 independent periodic boxes, each with 6 right-hand sides (unit
+    translation and rotation of sphere 0, the other spheres fixed), stepped
+    together: one launch per step over all problems, autotuned launch shape,
+    HIP-graph replay.  Reports throughput and the 6x6 resistance matrices of
+    sphere 0 after measure(); an FP32 batch is repeated in FP64 (same launch
+    shape class, not timed against it) and the matrices are compared."""
+    import numpy as np
+    boxes = make_boxes(args.batch_boxes, args.seed + 7)
+    res, mats = batch_run(L, args, args.precision, boxes, u0, tau, True)
+    eta = 72.0 * (tau - 0.5) / 3.0
+    drag = [-np.mean(np.diag(Rm)[:3]) / (6 * np.pi * eta * bx[1][0][0])
+            for Rm, bx in zip(mats, boxes)]
+    asym = [np.abs(Rm - Rm.T).max() / np.abs(Rm).max() for Rm in mats]
+    res["translational_drag_over_stokes_median"] = float(np.median(drag))
+    res["resistance_asymmetry_median"] = float(np.median(asym))
+    if args.precision == 32:
+        r64, mats64 = batch_run(L, args, 64, boxes, u0, tau, False)
+        diff = [np.abs(a - c).max() / np.abs(c).max() for a, c in zip(mats, mats64)]
+        res["fp64"] = {"ms_per_step": r64["ms_per_step"], "MLUPS": r64["MLUPS"]}
+        res["resistance_rel_diff_fp32_vs_fp64"] = {"max": float(np.max(diff)),
+                                                   "median": float(np.median(diff))}
     if not args.no_cpu_baseline:
         res["cpu_baseline"] = cpu_batch_baseline(boxes, args.cpu_batch_steps, u0, tau)
     log(f"batch: {res}")

@@ -6,11 +6,14 @@ oracle/Makefile (oracle/_ref/libslbref.so, FP64, and libslbref32.so,
 Run here, where the reference sources exist:
 
     make -C oracle ref && python tests/golden/make_golden.py
+    python tests/golden/make_golden.py packs      # only the workload-size tables
 
 Every lattice is non-cubic so that a transposed index cannot pass.  Body
 forces use the reference's current API (zerobf + addbf on fluid nodes).
 """
 import ctypes as C
+import gzip
+import hashlib
 import json
 import os
 import sys
@@ -498,8 +501,96 @@ def case_diagnostics(R):
     print("wrote diagnostics")
 
 
+def rsa_spheres(rng, dims, r_lo, r_hi, count, trials, gap=0.0):
+    """Random sequential addition in a periodic box: radius U[r_lo, r_hi],
+    centre uniform, accepted when the minimum-image distance to every placed
+    sphere is at least the sum of the radii plus gap."""
+    Ld = np.array(dims, dtype=np.float64)
+    placed = []
+    for _ in range(trials):
+        if len(placed) == count:
+            break
+        r = float(rng.uniform(r_lo, r_hi))
+        c = rng.uniform(0.0, 1.0, 3) * Ld
+        ok = True
+        for (r2, *c2) in placed:
+            d = c - np.array(c2)
+            d -= Ld * np.round(d / Ld)
+            if np.sqrt((d * d).sum()) < r + r2 + gap:
+                ok = False
+                break
+        if ok:
+            placed.append((r, *c))
+    return placed
+
+
+def image_inserts(dims, spheres):
+    """Each sphere and every periodic image whose bounding box meets the box,
+    in insertion order, one insertsph call (one object) each."""
+    out = []
+    for (r, *c) in spheres:
+        for a in (-1, 0, 1):
+            for b in (-1, 0, 1):
+                for e in (-1, 0, 1):
+                    o = (c[0] + a * dims[0], c[1] + b * dims[1], c[2] + e * dims[2])
+                    if all(-r < o[d] < dims[d] + r for d in range(3)):
+                        out.append((r, *o))
+    return np.array(out, dtype=np.float64).reshape(-1, 4)
+
+
+def sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a, dtype="<i4").tobytes()).hexdigest()
+
+
+def case_packs(R):
+    """Solid maps and boundary-link masks at workload sizes: a 128^3 pack of
+    radius-6 spheres (solid fraction about 0.25, then flattened to one
+    object), a non-cubic pack of mixed radii, and boxes like the batched ones
+    (edge 32..64, 2..20 spheres of radius 3..6).  The tables are stored as
+    SHA-256 digests of their little-endian int32 bytes with the counts, so
+    the fixture stays small while the comparison stays bit for bit."""
+    rng = np.random.default_rng(20261203)
+    cases = [("pack128", (128, 128, 128), rsa_spheres(rng, (128,) * 3, 6.0, 6.0, 580, 400000),
+              True),
+             ("pack_noncubic", (72, 60, 66), rsa_spheres(rng, (72, 60, 66), 3.0, 8.0, 60, 100000),
+              True)]
+    for q in range(6):
+        Lb = int(rng.integers(32, 65))
+        ns = int(rng.integers(2, 21))
+        cases.append((f"box{q}", (Lb,) * 3, rsa_spheres(rng, (Lb,) * 3, 3.0, 6.0, ns, 20000, 1.0),
+                      False))
+    out = {}
+    for name, dims, spheres, flatten in cases:
+        ins = image_inserts(dims, spheres)
+        h = R.new(*dims)
+        placed = [1 if R.L.insertsph(h, *row) else 0 for row in ins]
+        R.L.syncobjs(h)
+        st = state(R, h)
+        rec = dict(dims=list(dims), spheres=len(spheres), placed=int(sum(placed)),
+                   mask_sha256=sha(st["mask"]), objid_sha256=sha(st["objid"]),
+                   solid=int((st["objid"] >= 0).sum()),
+                   mask_nonzero=int((st["mask"] != 0).sum()),
+                   countfluid=int(R.L.countfluid(h)),
+                   countmask_links=int(R.L.countmask(h, b"links")),
+                   countmask_faces=int(R.L.countmask(h, b"faces")))
+        if flatten:
+            R.L.flattenobj(h)
+            R.L.syncobjs(h)
+            st = state(R, h)
+            rec["flat_mask_sha256"] = sha(st["mask"])
+            rec["flat_objid_sha256"] = sha(st["objid"])
+        out[name] = dict(record=rec, inserts=ins.tolist(), placed_list=placed)
+        print(name, {k: v for k, v in rec.items() if "sha" not in k})
+    with gzip.open(os.path.join(OUT, "packs.json.gz"), "wt") as f:
+        json.dump(out, f)
+    print("wrote packs")
+
+
 def main():
     R = RefLib(64)
+    if sys.argv[1:] == ["packs"]:
+        case_packs(R)
+        return
     R32 = RefLib(32)
     case_tables(R)
     case_static(R, "static_stokes", (12, 10, 14), [[3.3, 5.2, 4.7, 6.9]],
@@ -515,6 +606,7 @@ def main():
     case_sphere_c(R)
     case_images(R)
     case_diagnostics(R)
+    case_packs(R)
 
 
 if __name__ == "__main__":

@@ -320,3 +320,67 @@ def test_checkpoint_round_trip(tmp_path):
     lat.L.lbsavevel(lat.h, v, b"ascii")
     rows = open(v.decode()).read().split("\n")
     assert len(rows[0].split()) == 7 and rows[0].split()[4:] == ["0", "0", "0"]
+
+
+def _packs():
+    import gzip
+    with gzip.open(os.path.join(GOLDEN, "packs.json.gz"), "rt") as f:
+        return json.load(f)
+
+
+def _sha(a):
+    import hashlib
+    return hashlib.sha256(np.ascontiguousarray(a, dtype="<i4").tobytes()).hexdigest()
+
+
+def _dense_objid_fast(lat):
+    """dense_objid by a sorted lookup (packs have hundreds of objects)."""
+    s = lat.s
+    addrs = []
+    p = s.obj_head
+    while p:
+        addrs.append(C.addressof(p.contents))
+        p = p.contents.next_obj
+    o = np.ctypeslib.as_array(C.cast(s.obj, C.POINTER(C.c_uint64)), shape=(lat.N,))
+    out = np.full(lat.N, -1, dtype=np.int32)
+    if not addrs:
+        return out
+    a = np.array(addrs, dtype=np.uint64)
+    order = np.argsort(a)
+    pos = np.minimum(np.searchsorted(a[order], o), len(a) - 1)
+    hit = (o != 0) & (a[order][pos] == o)
+    out[hit] = order[pos[hit]].astype(np.int32)
+    assert (hit == (o != 0)).all()
+    return out
+
+
+@pytest.mark.parametrize("name", ["pack128", "pack_noncubic", "box0", "box1", "box2", "box3",
+                                  "box4", "box5"])
+def test_tables_at_workload_sizes_bit_exact(name):
+    """Workload-size geometries through the C API: the 128^3 radius-6 pack of
+    the passive-scalar workload, a non-cubic mixed-radius pack and boxes like
+    the batched ones.  Solid map (host) and boundary-link mask (computed on
+    the GPU) equal the reference library's bit for bit (SHA-256 of the int32
+    tables), before and after flattenobj, with countfluid and countmask."""
+    case = _packs()[name]
+    rec = case["record"]
+    lat = lattice(tuple(rec["dims"]))
+    L, h = lat.L, lat.h
+    placed = [1 if L.insertsph(h, *row) else 0 for row in case["inserts"]]
+    assert placed == case["placed_list"]
+    L.syncobjs(h)
+    dense = _dense_objid_fast(lat)
+    mask = lat.mask()
+    assert int((dense >= 0).sum()) == rec["solid"]
+    assert int((mask != 0).sum()) == rec["mask_nonzero"]
+    assert _sha(dense) == rec["objid_sha256"]
+    assert _sha(mask) == rec["mask_sha256"]
+    assert L.countfluid(h) == rec["countfluid"]
+    assert L.countmask(h, b"links") == rec["countmask_links"]
+    assert L.countmask(h, b"faces") == rec["countmask_faces"]
+    if "flat_mask_sha256" in rec:
+        L.flattenobj(h)
+        L.syncobjs(h)
+        assert _sha(_dense_objid_fast(lat)) == rec["flat_objid_sha256"]
+        assert _sha(lat.mask()) == rec["flat_mask_sha256"]
+    lat.free()

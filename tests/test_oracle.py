@@ -317,3 +317,45 @@ def test_mp_partial_adsorption_limits(oracle):
     assert abs(p.sum() - tot) < 1e-11 * tot
     ads = p[objid >= 0].sum()
     assert ads > 0 and ads < p1[objid >= 0].sum()
+
+
+def _packs():
+    import gzip
+    with gzip.open(os.path.join(GOLDEN, "packs.json.gz"), "rt") as f:
+        return json.load(f)
+
+
+def _sha(a):
+    import hashlib
+    return hashlib.sha256(np.ascontiguousarray(a, dtype="<i4").tobytes()).hexdigest()
+
+
+@pytest.mark.parametrize("name", ["pack128", "pack_noncubic", "box0", "box3"])
+def test_tables_at_workload_sizes(oracle, name):
+    """Solid map and boundary-link mask of workload-size packs (128^3 radius-6
+    pack, a non-cubic mixed-radius pack, batch-like boxes) equal the
+    reference library's bit for bit (SHA-256 of the int32 tables)."""
+    case = _packs()[name]
+    rec = case["record"]
+    dims = tuple(rec["dims"])
+    N = int(np.prod(dims))
+    objid = np.full(N, -1, dtype=np.int32)
+    placed = []
+    for k, row in enumerate(case["inserts"]):
+        ns, _ = oracle.insertsph(objid, dims, k, *row)
+        placed.append(1 if ns else 0)
+    assert placed == case["placed_list"]
+    # reference numbering: list order, newest placed object first
+    kept = [k for k, p in enumerate(placed) if p][::-1]
+    remap = np.full(len(placed), -1, dtype=np.int32)
+    remap[kept] = np.arange(len(kept), dtype=np.int32)
+    dense = np.where(objid >= 0, remap[np.maximum(objid, 0)], -1)
+    mask = oracle.makemask(dense, dims)
+    assert int((dense >= 0).sum()) == rec["solid"]
+    assert int((mask != 0).sum()) == rec["mask_nonzero"]
+    assert _sha(dense) == rec["objid_sha256"]
+    assert _sha(mask) == rec["mask_sha256"]
+    if "flat_mask_sha256" in rec:
+        flat = np.where(dense >= 0, 0, -1)
+        assert _sha(flat) == rec["flat_objid_sha256"]
+        assert _sha(oracle.makemask(flat, dims)) == rec["flat_mask_sha256"]
