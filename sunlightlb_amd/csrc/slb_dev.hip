@@ -554,8 +554,12 @@ struct MPArgs {
  *   pi_l(y) = a_l(y)/rho(y) + (w_l/72)(rho0/rho(y) - alpha) (+ alpha, l = 0)
  * then the update rule of src/mpkernel.c:190-197 (fluid replace, adsorbing
  * site accumulate, other solid clear; partial adsorption keeps pads*new). */
+/* FP64 records are 32 bytes: capped at 4 waves per SIMD, the compiler keeps the
+ * whole gather in flight instead of interleaving loads and waits to hold
+ * eight waves' worth of registers (the FP32 kernel does both) */
 template <typename T, int NMOM, bool MOVING, typename I>
 __global__ void __launch_bounds__(256)
+__attribute__((amdgpu_waves_per_eu(1, sizeof(T) == 8 ? 4 : 8)))
 k_mpstep(const T *__restrict__ a, const T *__restrict__ qin, T *__restrict__ qout,
          const uint32_t *__restrict__ mpf, const int32_t *__restrict__ mask,
          const int32_t *__restrict__ objidx, const double *__restrict__ tab,
