@@ -141,7 +141,7 @@ k_bbl(T *__restrict__ a, const int32_t *__restrict__ mask,
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= G.plane) return;
   const int i = blockIdx.y, j = t / G.esk, k = t - j * G.esk;
-  const long x = (long)i * G.plane + t, N = G.N;
+  const long x = (long)i * G.plane + t;
   const int m = mask[x];
   if (m <= 0) return;
   const AxisOff oi = axis_off(i, G.esi, G.plane);
@@ -178,7 +178,7 @@ k_propagate(const T *__restrict__ src, T *__restrict__ dst, Geom G) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= G.plane) return;
   const int i = blockIdx.y, j = t / G.esk, k = t - j * G.esk;
-  const long x = (long)i * G.plane + t, N = G.N;
+  const long x = (long)i * G.plane + t;
   const AxisOff oi = axis_off(i, G.esi, G.plane);
   const AxisOff oj = axis_off(j, G.esj, G.esk);
   const AxisOff ok = axis_off(k, G.esk, 1);
@@ -199,7 +199,6 @@ k_collide(T *__restrict__ a, const uint16_t *__restrict__ flags,
   if (x >= G.N) return;
   const bool solid = (flags[x] & FLAG_SOLID) != 0;
   if (P.skip_solid && solid) return;
-  const long N = G.N;
   T v[15];
 #pragma unroll
   for (int l = 0; l < 15; l++) v[l] = a[pidx(l, x)];

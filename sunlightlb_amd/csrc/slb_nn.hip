@@ -522,7 +522,7 @@ k_bias_grad(const uint16_t *__restrict__ dP, int C, int N, float *__restrict__ d
     s += bf2f(dP[n * C + c]);
   sm[threadIdx.x] = s;
   __syncthreads();
-  if (threadIdx.x < C) {
+  if ((int)threadIdx.x < C) {
     float t = 0.f;
     for (int k = 0; k < R; k++) t += sm[k * C + threadIdx.x];
     atomicAdd(db + threadIdx.x, t);
@@ -788,7 +788,7 @@ k_bias_grad8(const uint16_t *__restrict__ dP, int C, int N, float *__restrict__ 
 #pragma unroll
   for (int q = 0; q < 8; q++) sm[threadIdx.x * 8 + q] = acc[q];
   __syncthreads();
-  if (threadIdx.x < C) {
+  if ((int)threadIdx.x < C) {
     const int gg = threadIdx.x >> 3, q = threadIdx.x & 7;
     float t = 0.f;
     for (int k = 0; k < R; k++) t += sm[(k * G8 + gg) * 8 + q];
