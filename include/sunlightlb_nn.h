@@ -27,9 +27,13 @@ void slbn_set_stream(void *stream);
 void slbn_conv(const uint16_t *X, int cin, const uint16_t *W, const float *bias, uint16_t *Y,
                float *Yf, int cout, int relu, int transposed, int D, int H, int Wd);
 /* dW[co][t*cin+ci] += sum_n dY[n][co] X[n+off(t)][ci]; channels 16..64 step
- * 16.  When slbn_wgrad_blocked_ok() (x extent a multiple of 8, channels 16,
- * 32 or 64) it is a GEMM over a node-blocked copy of X and dY with a fixed
- * summation order; otherwise a per-tap LDS-staged kernel with atomics. */
+ * 16.  When slbn_wgrad_lds16_ok() (16 -> 16 channels, x extent a multiple of
+ * 8 up to 128, even y extent) bricks of two x-rows are staged in LDS and all
+ * 27 taps computed there; else when slbn_wgrad_blocked_ok() (x extent a
+ * multiple of 8, channels 16, 32 or 64) it is a GEMM over a node-blocked copy
+ * of X and dY; both with a fixed summation order.  Otherwise a per-tap
+ * LDS-staged kernel with atomics. */
+int  slbn_wgrad_lds16_ok(int cin, int cout, int D, int H, int Wd);
 int  slbn_wgrad_blocked_ok(int cin, int cout, int D, int H, int Wd);
 void slbn_conv_wgrad(const uint16_t *X, int cin, const uint16_t *dY, int cout, float *dW, int D,
                      int H, int Wd);

@@ -208,6 +208,7 @@ SIGNATURES = {
     "slbn_conv_wgrad": (None, [VP, I, VP, I, VP, I, I, I]),
     "slbn_conv_wgrad_small": (None, [VP, I, VP, I, VP, I, I, I]),
     "slbn_wgrad_blocked_ok": (I, [I, I, I, I, I]),
+    "slbn_wgrad_lds16_ok": (I, [I, I, I, I, I]),
     "slbn_wtrans": (None, [VP, I, I, VP]),
     "slbn_relu_bwd": (None, [VP, VP, VP, C.c_long]),
     "slbn_bias_grad": (None, [VP, I, I, VP]),

@@ -456,6 +456,145 @@ k_wgrad_b(const uint16_t *__restrict__ Xb, const uint16_t *__restrict__ Db,
   }
 }
 
+/* Weight gradient of the 16-to-16 full-resolution layers, staged in LDS.
+ * A brick is two x-rows (y0, y0 + 1) of plane z.  The block stages the 12 input
+ * rows (z - 1 .. z + 1) x (y0 - 1 .. y0 + 2) and the 2 output-gradient rows
+ * transposed to [row][channel][x] (channel rows padded by 8 elements, so the
+ * 16 channels of one read fall in different banks), and all 27 taps read them
+ * there: the MFMA operands are 8 consecutive nodes of one channel (aligned
+ * 16-byte reads), the x shifts of +-1 a funnel with one dword of the
+ * neighbouring group, wrapping periodically within the row.  Wave w takes
+ * 32-node chunks w, w + 4, ... of each brick and keeps all 27 taps'
+ * accumulators; a block walks a contiguous range of bricks (blocks on one XCD
+ * take adjacent ranges), then reduces its 4 waves in LDS and writes
+ * part[t][block][co][ci] for k_wgrad_reduce.  Requires Wd % 8 == 0,
+ * Wd <= 128 and H even. */
+#define WG16_SW(Wd) ((Wd) + 8)
+__global__ void __launch_bounds__(256)
+k_wgrad16_lds(const uint16_t *__restrict__ X, const uint16_t *__restrict__ dY,
+              float *__restrict__ part, int per, NGeom G) {
+  extern __shared__ __align__(16) uint16_t lds[];
+  const int Wd = G.Wd, SW = WG16_SW(G.Wd), hb = G.H >> 1;
+  const int nbricks = G.D * hb;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, grp = lane >> 4, m = lane & 15;
+  /* blocks b, b + 8, ... run on one XCD: give them adjacent brick ranges */
+  int b = blockIdx.x;
+  if ((gridDim.x & 7) == 0) b = (b & 7) * (gridDim.x >> 3) + (b >> 3);
+  const int kb = b * per, ke = min(nbricks, kb + per);
+  uint16_t *Xs = lds;                       /* [12][16][SW] */
+  uint16_t *Ds = lds + 12 * 16 * SW;        /* [2][16][SW] */
+  const int gpr = Wd >> 3;                  /* 8-node groups per row */
+  const int cpr = (gpr + 3) >> 2;           /* 32-node chunks per row */
+  f32x4 acc[27];
+#pragma unroll
+  for (int t = 0; t < 27; t++) acc[t] = (f32x4){0.f, 0.f, 0.f, 0.f};
+  for (int k = kb; k < ke; k++) {
+    const int z = k / hb, y0 = 2 * (k - z * hb);
+    __syncthreads();                        /* the previous brick is consumed */
+    const int pairs = Wd >> 1;
+    for (int p = threadIdx.x; p < 14 * pairs; p += 256) {
+      const int r = p / pairs, x = 2 * (p - r * pairs);
+      int zz, yy;
+      const uint16_t *src;
+      uint16_t *dst;
+      if (r < 12) {
+        zz = z + r / 4 - 1;
+        yy = y0 + (r & 3) - 1;
+        src = X;
+        dst = Xs + r * 16 * SW;
+      } else {
+        zz = z;
+        yy = y0 + (r - 12);
+        src = dY;
+        dst = Ds + (r - 12) * 16 * SW;
+      }
+      zz += (zz < 0) ? G.D : 0; zz -= (zz >= G.D) ? G.D : 0;
+      yy += (yy < 0) ? G.H : 0; yy -= (yy >= G.H) ? G.H : 0;
+      const long n = ((long)zz * G.H + yy) * Wd + x;
+      const u32x4 a0 = *(const u32x4 *)(src + n * 16);
+      const u32x4 a1 = *(const u32x4 *)(src + n * 16 + 8);
+      const u32x4 b0 = *(const u32x4 *)(src + (n + 1) * 16);
+      const u32x4 b1 = *(const u32x4 *)(src + (n + 1) * 16 + 8);
+      const uint32_t av[8] = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w};
+      const uint32_t bv[8] = {b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, b1.z, b1.w};
+#pragma unroll
+      for (int q = 0; q < 8; q++) {
+        /* channel 2q: low halves of nodes x, x + 1; channel 2q + 1: high halves */
+        *(uint32_t *)(dst + (2 * q) * SW + x) = (av[q] & 0xffffu) | (bv[q] << 16);
+        *(uint32_t *)(dst + (2 * q + 1) * SW + x) = (av[q] >> 16) | (bv[q] & 0xffff0000u);
+      }
+    }
+    __syncthreads();
+    for (int c = wv; c < 2 * cpr; c += 4) {
+      const int q = c / cpr, gx = 4 * (c - q * cpr) + grp;
+      const bool ok = gx < gpr;
+      const int x0 = ok ? 8 * gx : 0;
+      const int xp = (x0 == 0) ? Wd - 2 : x0 - 2;      /* dword holding node x0 - 1 */
+      const int xn = (x0 + 8 == Wd) ? 0 : x0 + 8;      /* dword holding node x0 + 8 */
+      u32x4 av = *(const u32x4 *)(Ds + (q * 16 + m) * SW + x0);
+      if (!ok) av = (u32x4){0u, 0u, 0u, 0u};
+      const bf16x8 A = as_bf(av);
+#pragma unroll
+      for (int pr = 0; pr < 9; pr++) {
+        const int dz = pr / 3, dy = pr % 3;             /* offsets + 1 */
+        const uint16_t *row = Xs + ((dz * 4 + q + dy) * 16 + m) * SW;
+        const u32x4 v0 = *(const u32x4 *)(row + x0);
+        const uint32_t vm = *(const uint32_t *)(row + xp);
+        const uint32_t vp = *(const uint32_t *)(row + xn);
+        u32x4 bm, bp;
+        /* nodes x0 - 1 .. x0 + 6 */
+        bm.x = (vm >> 16) | (v0.x << 16);
+        bm.y = (v0.x >> 16) | (v0.y << 16);
+        bm.z = (v0.y >> 16) | (v0.z << 16);
+        bm.w = (v0.z >> 16) | (v0.w << 16);
+        /* nodes x0 + 1 .. x0 + 8 */
+        bp.x = (v0.x >> 16) | (v0.y << 16);
+        bp.y = (v0.y >> 16) | (v0.z << 16);
+        bp.z = (v0.z >> 16) | (v0.w << 16);
+        bp.w = (v0.w >> 16) | (vp << 16);
+        const int t = pr * 3;
+        acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, as_bf(bm), acc[t], 0, 0, 0);
+        acc[t + 1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, as_bf(v0), acc[t + 1], 0, 0, 0);
+        acc[t + 2] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, as_bf(bp), acc[t + 2], 0, 0, 0);
+      }
+    }
+  }
+  /* reduce the 4 waves in three rounds of 9 taps: waves 1..3 write, wave 0 adds */
+  float *red = (float *)lds;                /* [3][9 * 4][64] */
+#pragma unroll
+  for (int rd = 0; rd < 3; rd++) {
+    __syncthreads();
+    if (wv > 0) {
+#pragma unroll
+      for (int u = 0; u < 9; u++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) red[((wv - 1) * 36 + u * 4 + r) * 64 + lane] = acc[9 * rd + u][r];
+    }
+    __syncthreads();
+    if (wv == 0) {
+#pragma unroll
+      for (int u = 0; u < 9; u++) {
+        const int t = 9 * rd + u;
+        float *out = part + ((long)t * gridDim.x + blockIdx.x) * 256;
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          const int qq = u * 4 + r;
+          const float v = acc[t][r] + red[qq * 64 + lane] + red[(36 + qq) * 64 + lane] +
+                          red[(72 + qq) * 64 + lane];
+          /* lane holds D[co = 4 grp + r][ci = m] */
+          out[(4 * grp + r) * 16 + m] = v;
+        }
+      }
+    }
+  }
+}
+
+static size_t wg16_lds_bytes(int Wd) {
+  const size_t stage = (size_t)14 * 16 * WG16_SW(Wd) * sizeof(uint16_t);
+  const size_t red = (size_t)3 * 36 * 64 * sizeof(float);
+  return stage > red ? stage : red;
+}
+
 /* dW[co][t*Cin + ci] += sum over blocks b (in order) of part[t][b][co][ci] */
 /* Block of 256 threads = 32 outputs x 8 strands: strand s sums the partials
  * b = s, s + 8, ... in order, then the 8 strand sums are added in order
@@ -900,6 +1039,14 @@ static void wgrad_staged(const uint16_t *X, int cin, const uint16_t *dY, int cou
   }
 }
 
+/* the LDS-staged 16 -> 16 weight gradient applies (SLBN_WGRAD_LDS=0 turns it off) */
+extern "C" int slbn_wgrad_lds16_ok(int cin, int cout, int D, int H, int Wd) {
+  static int env = -1;
+  if (env < 0) env = getenv("SLBN_WGRAD_LDS") ? atoi(getenv("SLBN_WGRAD_LDS")) : 1;
+  return env != 0 && cin == 16 && cout == 16 && D >= 1 && H >= 2 && H % 2 == 0 &&
+         Wd >= 8 && Wd % 8 == 0 && Wd <= 128;
+}
+
 extern "C" int slbn_wgrad_blocked_ok(int cin, int cout, int D, int H, int Wd) {
   (void)D; (void)H;
   return Wd % 8 == 0 && (cin == 16 || cin == 32 || cin == 64) &&
@@ -916,6 +1063,21 @@ extern "C" void slbn_conv_wgrad(const uint16_t *X, int cin, const uint16_t *dY, 
   }
   if (!slbn_wgrad_blocked_ok(cin, cout, D, H, Wd)) {
     wgrad_staged(X, cin, dY, cout, dW, G);
+    return;
+  }
+  if (slbn_wgrad_lds16_ok(cin, cout, D, H, Wd)) {
+    /* 16 -> 16 channels: LDS-staged bricks, about two blocks per CU */
+    const int nbricks = D * (H / 2);
+    int nb = nbricks < 512 ? nbricks : 512;
+    const int per = (nbricks + nb - 1) / nb;
+    nb = (nbricks + per - 1) / per;
+    float *part = (float *)wg_workspace((size_t)27 * nb * 256 * sizeof(float));
+    hipLaunchKernelGGL(k_wgrad16_lds, dim3((unsigned)nb), dim3(256), wg16_lds_bytes(Wd), nn_stream,
+                       X, dY, part, per, G);
+    LAUNCHCHECK();
+    hipLaunchKernelGGL(k_wgrad_reduce, dim3(blocks_for(27L * 256, 32)), dim3(256), 0, nn_stream,
+                       part, nb, 16, 16, Kp, dW);
+    LAUNCHCHECK();
     return;
   }
   const long ngr = (long)G.N >> 3;

@@ -141,10 +141,13 @@ def test_conv_input_gradient_matches_oracle(shape, cin, cout):
                                                   ((32, 32, 32), 64, 64, 0), ((16, 8, 8), 64, 32, 0),
                                                   ((8, 8, 12), 16, 32, 0), ((4, 6, 6), 64, 16, 0),
                                                   ((8, 16, 16), 32, 16, 0),
+                                                  ((6, 10, 24), 16, 16, 0), ((3, 2, 8), 16, 16, 0),
+                                                  ((5, 6, 128), 16, 16, 0), ((4, 8, 40), 16, 16, 0),
                                                   ((8, 8, 12), 8, 16, 1), ((6, 8, 10), 16, 8, 1)])
 def test_conv_weight_gradient_matches_oracle(shape, cin, cout, small):
     L = lib()
     print(f"conv wgrad {shape} {cin}->{cout} small={small} "
+          f"lds16={L.slbn_wgrad_lds16_ok(cin, cout, *shape)} "
           f"blocked={L.slbn_wgrad_blocked_ok(cin, cout, *shape)}", flush=True)
     rng = np.random.default_rng(30 + cin + cout)
     xb, x = bf_random(rng, shape + (cin,))
