@@ -162,25 +162,29 @@ k_conv(const uint16_t *__restrict__ X, int Cin, const uint16_t *__restrict__ W, 
 
 /* The same convolution for CIN = 16, 32 or 64 input channels with the input
  * brick staged in LDS: a block computes XT = 1024/CIN x-nodes x 4 rows of
- * one plane (wave w: row w, XT/16 tiles of 16 nodes) and first loads the
- * (XT+2) x 6 x 3 halo brick (38-41 KB, periodic wrap applied at load) once,
+ * ZT planes (wave w: row w, XT/16 tiles of 16 nodes per plane) and first
+ * loads the (XT+2) x 6 x (ZT+2) halo brick (38-41 KB for one plane, 51 KB for
+ * two; periodic wrap applied at load) once,
  * so the 27 taps read the input from LDS instead of 27 times through L2.
  * The K loop, MFMA order and epilogue are those of k_conv: results are
  * bit-identical.  Needs Wd % XT == 0 and H % 4 == 0. */
 #define CL_TY 6
-#define CL_TZ 3
-template <int CIN, int MT>
+/* ZT = 2: the block computes planes z0 and z0 + 1 from one (XT+2) x 6 x 4 brick
+ * (the input of the four planes z0 - 1 .. z0 + 2 staged once for both, 3x the
+ * output nodes instead of 4.5x), each chunk's weights applied to both planes;
+ * every output tile keeps k_conv's K order and MFMA sequence.  Needs D even. */
+template <int CIN, int MT, int ZT>
 __global__ void __launch_bounds__(256)
 k_conv_lds(const uint16_t *__restrict__ X, const uint16_t *__restrict__ W, int Kp,
            const float *__restrict__ bias, uint16_t *__restrict__ Y, float *__restrict__ Yf,
            int Cout, int relu, int sgn, NGeom G) {
-  constexpr int XT = 1024 / CIN, NTW = XT / 16, TX = XT + 2, CH = CIN / 8;
-  __shared__ __attribute__((aligned(16))) uint16_t sX[CL_TZ * CL_TY * TX * CIN];
+  constexpr int XT = 1024 / CIN, NTW = XT / 16, TX = XT + 2, CH = CIN / 8, TZ = ZT + 2;
+  __shared__ __attribute__((aligned(16))) uint16_t sX[TZ * CL_TY * TX * CIN];
   const int nbx = G.Wd / XT, nby = G.H >> 2;
   const int bx = blockIdx.x % nbx, rest = blockIdx.x / nbx;
-  const int by = rest % nby, z = rest / nby;
+  const int by = rest % nby, z = (rest / nby) * ZT;
   const int x0 = bx * XT, y0 = by * 4;
-  for (int e = threadIdx.x; e < CL_TZ * CL_TY * TX * CH; e += blockDim.x) {
+  for (int e = threadIdx.x; e < TZ * CL_TY * TX * CH; e += blockDim.x) {
     const int part = e % CH, node = e / CH;
     const int lx = node % TX, r = node / TX, ly = r % CL_TY, lz = r / CL_TY;
     int gx = x0 + lx - 1, gy = y0 + ly - 1, gz = z + lz - 1;
@@ -193,11 +197,13 @@ k_conv_lds(const uint16_t *__restrict__ X, const uint16_t *__restrict__ W, int K
   __syncthreads();
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, grp = lane >> 4, nl = lane & 15;
   constexpr int K = 27 * CIN;
-  f32x4 acc[NTW][MT];
+  f32x4 acc[ZT][NTW][MT];
 #pragma unroll
-  for (int u = 0; u < NTW; u++)
+  for (int zp = 0; zp < ZT; zp++)
 #pragma unroll
-    for (int m = 0; m < MT; m++) acc[u][m] = (f32x4){0.f, 0.f, 0.f, 0.f};
+    for (int u = 0; u < NTW; u++)
+#pragma unroll
+      for (int m = 0; m < MT; m++) acc[zp][u][m] = (f32x4){0.f, 0.f, 0.f, 0.f};
   for (int k0 = 0; k0 < Kp; k0 += 32) {
     const int kk = k0 + 8 * grp;
     bf16x8 a[MT];
@@ -207,27 +213,37 @@ k_conv_lds(const uint16_t *__restrict__ X, const uint16_t *__restrict__ W, int K
       a[m] = (bf16x8){0, 0, 0, 0, 0, 0, 0, 0};
       if (co < Cout) a[m] = *(const bf16x8 *)(W + (long)co * Kp + kk);
     }
-    bf16x8 b[NTW];
+    bf16x8 b[ZT][NTW];
     if (kk < K) {
       const int t = kk / CIN, ci = kk % CIN;
       const int dz = (t / 9 - 1) * sgn, dy = ((t / 3) % 3 - 1) * sgn, dx = (t % 3 - 1) * sgn;
-      const int base = ((1 + dz) * CL_TY + (wv + 1 + dy)) * TX + 1 + dx + nl;
 #pragma unroll
-      for (int u = 0; u < NTW; u++)
-        b[u] = as_bf(*(const u32x4 *)(sX + (base + 16 * u) * CIN + ci));
+      for (int zp = 0; zp < ZT; zp++) {
+        const int base = ((1 + dz + zp) * CL_TY + (wv + 1 + dy)) * TX + 1 + dx + nl;
+#pragma unroll
+        for (int u = 0; u < NTW; u++)
+          b[zp][u] = as_bf(*(const u32x4 *)(sX + (base + 16 * u) * CIN + ci));
+      }
     } else {
 #pragma unroll
-      for (int u = 0; u < NTW; u++) b[u] = (bf16x8){0, 0, 0, 0, 0, 0, 0, 0};
+      for (int zp = 0; zp < ZT; zp++)
+#pragma unroll
+        for (int u = 0; u < NTW; u++) b[zp][u] = (bf16x8){0, 0, 0, 0, 0, 0, 0, 0};
     }
 #pragma unroll
-    for (int u = 0; u < NTW; u++)
+    for (int zp = 0; zp < ZT; zp++)
 #pragma unroll
-      for (int m = 0; m < MT; m++)
-        acc[u][m] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[m], b[u], acc[u][m], 0, 0, 0);
+      for (int u = 0; u < NTW; u++)
+#pragma unroll
+        for (int m = 0; m < MT; m++)
+          acc[zp][u][m] =
+              __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[m], b[zp][u], acc[zp][u][m], 0, 0, 0);
   }
 #pragma unroll
+  for (int zp = 0; zp < ZT; zp++)
+#pragma unroll
   for (int u = 0; u < NTW; u++) {
-    const int n = (z * G.H + y0 + wv) * G.Wd + x0 + 16 * u + nl;
+    const int n = ((z + zp) * G.H + y0 + wv) * G.Wd + x0 + 16 * u + nl;
 #pragma unroll
     for (int m = 0; m < MT; m++) {
       const int cb = 16 * m + 4 * grp;
@@ -235,7 +251,7 @@ k_conv_lds(const uint16_t *__restrict__ X, const uint16_t *__restrict__ W, int K
       float v[4];
 #pragma unroll
       for (int r = 0; r < 4; r++) {
-        v[r] = acc[u][m][r] + (bias ? bias[cb + r] : 0.f);
+        v[r] = acc[zp][u][m][r] + (bias ? bias[cb + r] : 0.f);
         if (relu) v[r] = v[r] > 0.f ? v[r] : 0.f;
       }
       if (Y) {
@@ -976,14 +992,18 @@ extern "C" void slbn_conv(const uint16_t *X, int cin, const uint16_t *W, const f
   const int xt = cin >= 16 ? 1024 / cin : 0;
   if ((cin == 16 || cin == 32 || cin == 64) && Wd % xt == 0 && H % 4 == 0 && mt <= 4 &&
       !(lds_env && atoi(lds_env) == 0)) {
-    const dim3 gl((unsigned)((Wd / xt) * (H / 4) * D));
-#define SLBN_CL(C, MT)                                                                      \
-  hipLaunchKernelGGL((k_conv_lds<C, MT>), gl, dim3(256), 0, nn_stream, X, W, Kp, bias, Y, Yf, \
-                     cout, relu, sgn, G)
+    /* two planes per block for the thin layers (cin 16, at most 32 outputs) */
+    const int zt = (cin == 16 && mt <= 2 && D % 2 == 0) ? 2 : 1;
+    const dim3 gl((unsigned)((Wd / xt) * (H / 4) * (D / zt)));
+#define SLBN_CL(C, MT, ZT)                                                                  \
+  hipLaunchKernelGGL((k_conv_lds<C, MT, ZT>), gl, dim3(256), 0, nn_stream, X, W, Kp, bias, Y, \
+                     Yf, cout, relu, sgn, G)
 #define SLBN_CLM(C)                                                                         \
-  if (mt == 1) SLBN_CL(C, 1); else if (mt == 2) SLBN_CL(C, 2);                              \
-  else if (mt == 3) SLBN_CL(C, 3); else SLBN_CL(C, 4);
-    if (cin == 16) { SLBN_CLM(16) }
+  if (mt == 1) SLBN_CL(C, 1, 1); else if (mt == 2) SLBN_CL(C, 2, 1);                        \
+  else if (mt == 3) SLBN_CL(C, 3, 1); else SLBN_CL(C, 4, 1);
+    if (cin == 16 && zt == 2) {
+      if (mt == 1) SLBN_CL(16, 1, 2); else SLBN_CL(16, 2, 2);
+    } else if (cin == 16) { SLBN_CLM(16) }
     else if (cin == 32) { SLBN_CLM(32) }
     else { SLBN_CLM(64) }
 #undef SLBN_CLM
