@@ -473,8 +473,9 @@ k_wgrad_b(const uint16_t *__restrict__ Xb, const uint16_t *__restrict__ Db,
 }
 
 /* Weight gradient of the 16-to-16 full-resolution layers, staged in LDS.
- * A brick is two x-rows (y0, y0 + 1) of plane z.  The block stages the 12 input
- * rows (z - 1 .. z + 1) x (y0 - 1 .. y0 + 2) and the 2 output-gradient rows
+ * A brick is two x-rows (y0, y0 + 1) of plane z.  The block holds the 12 input
+ * rows (z - 1 .. z + 1) x (y0 - 1 .. y0 + 2), 6 of them carried over from the
+ * previous brick of the same plane, and stages the 2 output-gradient rows
  * transposed to [row][channel][x] (channel rows padded by 8 elements, so the
  * 16 channels of one read fall in different banks), and all 27 taps read them
  * there: the MFMA operands are 8 consecutive nodes of one channel (aligned
@@ -504,25 +505,34 @@ k_wgrad16_lds(const uint16_t *__restrict__ X, const uint16_t *__restrict__ dY,
   f32x4 acc[27];
 #pragma unroll
   for (int t = 0; t < 27; t++) acc[t] = (f32x4){0.f, 0.f, 0.f, 0.f};
+  int zprev = -1;
   for (int k = kb; k < ke; k++) {
     const int z = k / hb, y0 = 2 * (k - z * hb);
+    /* input rows y0 - 1 .. y0 + 2 of each plane sit in ring slots (y + 4) & 3
+     * (y unwrapped within the plane): the next brick of the same plane keeps
+     * rows y0 + 1, y0 + 2 and loads y0 + 3, y0 + 4 into the two freed slots */
+    const bool step = (z == zprev);
+    zprev = z;
+    const int nx = step ? 6 : 12;
     __syncthreads();                        /* the previous brick is consumed */
     const int pairs = Wd >> 1;
-    for (int p = threadIdx.x; p < 14 * pairs; p += 256) {
+    for (int p = threadIdx.x; p < (nx + 2) * pairs; p += 256) {
       const int r = p / pairs, x = 2 * (p - r * pairs);
       int zz, yy;
       const uint16_t *src;
       uint16_t *dst;
-      if (r < 12) {
-        zz = z + r / 4 - 1;
-        yy = y0 + (r & 3) - 1;
+      if (r < nx) {
+        const int pl = step ? r >> 1 : r >> 2;
+        const int yu = step ? y0 + 1 + (r & 1) : y0 - 1 + (r & 3);
+        zz = z + pl - 1;
+        yy = yu;
         src = X;
-        dst = Xs + r * 16 * SW;
+        dst = Xs + (pl * 4 + ((yu + 4) & 3)) * 16 * SW;
       } else {
         zz = z;
-        yy = y0 + (r - 12);
+        yy = y0 + (r - nx);
         src = dY;
-        dst = Ds + (r - 12) * 16 * SW;
+        dst = Ds + (r - nx) * 16 * SW;
       }
       zz += (zz < 0) ? G.D : 0; zz -= (zz >= G.D) ? G.D : 0;
       yy += (yy < 0) ? G.H : 0; yy -= (yy >= G.H) ? G.H : 0;
@@ -553,7 +563,8 @@ k_wgrad16_lds(const uint16_t *__restrict__ X, const uint16_t *__restrict__ dY,
 #pragma unroll
       for (int pr = 0; pr < 9; pr++) {
         const int dz = pr / 3, dy = pr % 3;             /* offsets + 1 */
-        const uint16_t *row = Xs + ((dz * 4 + q + dy) * 16 + m) * SW;
+        const int slot = (y0 + q + dy - 1 + 4) & 3;     /* ring slot of row y0 + q + dy - 1 */
+        const uint16_t *row = Xs + ((dz * 4 + slot) * 16 + m) * SW;
         const u32x4 v0 = *(const u32x4 *)(row + x0);
         const uint32_t vm = *(const uint32_t *)(row + xp);
         const uint32_t vp = *(const uint32_t *)(row + xn);
