@@ -147,7 +147,8 @@ k_bstep(const T *__restrict__ src, T *__restrict__ dst,
         collide_node<T>(v, jbx, jby, jbz, A);
       }
 #pragma unroll
-      for (int l = 0; l < 15; l++) d[pidx32(l, x)] = v[l];
+      /* non-temporal: a batch step streams tens of GB (8.95 -> 8.59 ms at 512 boxes) */
+      for (int l = 0; l < 15; l++) __builtin_nontemporal_store(v[l], &d[pidx32(l, x)]);
     }
     if (lds) {
       __syncthreads();

@@ -661,6 +661,9 @@ k_mpstep(const T *__restrict__ a, const T *__restrict__ qin, T *__restrict__ qou
       if (NMOM > 1) { o[1] = qx[1] + frac * n1; o[2] = qx[2] + frac * n2; }
     }
   }
+  /* a plain store: the next step gathers these records from 15 neighbours,
+   * and at MP sizes they stay in the caches (non-temporal stores: 38 -> 45 us
+   * per step at 128^3) */
   ((V *)qout)[x] = o;
 }
 
