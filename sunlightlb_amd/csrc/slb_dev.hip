@@ -734,6 +734,7 @@ __global__ void k_scale(T *__restrict__ d, long n, double s) {
 
 struct slbd_lattice {
   int esi, esj, esk, prec;
+  int idx32;   /* population indices fit 32 bits (and SLB_INDEX64 does not force 64) */
   long rho_version;   /* bumped whenever rho changes (MP 1/rho slots) */
   long N;
   Geom G;
@@ -822,6 +823,8 @@ extern "C" slbd_lattice *slbd_create(int esi, int esj, int esk, int prec) {
   slbd_lattice *L = (slbd_lattice *)calloc(1, sizeof(slbd_lattice));
   L->esi = esi; L->esj = esj; L->esk = esk; L->prec = prec;
   L->N = (long)esi * esj * esk;
+  L->idx32 = 15 * pop_nodes(L->N) <= 0x7fffffffL &&
+             !(getenv("SLB_INDEX64") && atoi(getenv("SLB_INDEX64")) != 0);
   L->G.esi = esi; L->G.esj = esj; L->G.esk = esk; L->G.plane = esj * esk; L->G.N = L->N;
   L->G.nt_store = getenv("SLB_NT_STORE") ? atoi(getenv("SLB_NT_STORE")) : 1;
   L->G.nt_load = getenv("SLB_NT_LOAD") ? atoi(getenv("SLB_NT_LOAD")) : 0;
@@ -1328,7 +1331,7 @@ static void launch_step_t(slbd_lattice *L, const PhysArgs &A) {
   const T *src = (const T *)L->pops[L->cur];
   T *dst = (T *)L->pops[L->cur ^ 1];
   const dim3 grid((unsigned)((L->G.plane + L->G.step_tpb - 1) / L->G.step_tpb), (unsigned)L->esi);
-  if (15 * pop_nodes(L->G.N) <= 0x7fffffffL)
+  if (L->idx32)
     hipLaunchKernelGGL((k_step<T, MOVING, FORCE, BF, int>), grid, dim3(L->G.step_tpb), 0,
                        L->stream, src, dst, L->flags, L->ux, L->uy, L->uz, (const T *)L->bx,
                        (const T *)L->by, (const T *)L->bz, L->fx, L->fy, L->fz, L->G, A);
@@ -1724,7 +1727,7 @@ template <typename T, int NMOM, bool MOVING>
 static void launch_mp_t(slbd_lattice *L, slbd_mp *M, const MPArgs &A) {
   const int c = M->cur;
   /* 32-bit indices pay off (fewer VGPRs) only without the wall term */
-  if (!MOVING && 15 * pop_nodes(L->G.N) <= 0x7fffffffL)
+  if (!MOVING && L->idx32)
     hipLaunchKernelGGL((k_mpstep<T, NMOM, MOVING, int>), plane_grid(L), dim3(256), 0, L->stream,
                        (const T *)L->pops[L->cur], (const T *)M->q[c], (T *)M->q[c ^ 1], L->mpf,
                        L->mask, L->objidx, L->tab, L->G, A);

@@ -384,3 +384,52 @@ def test_tables_at_workload_sizes_bit_exact(name):
         assert _sha(_dense_objid_fast(lat)) == rec["flat_objid_sha256"]
         assert _sha(lat.mask()) == rec["flat_mask_sha256"]
     lat.free()
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+def test_64bit_index_path_is_bit_identical(prec, monkeypatch):
+    """Lattices of more than 2^31 populations index with 64-bit integers;
+    SLB_INDEX64=1 forces that path on a small lattice.  A Navier-Stokes,
+    skip-solid run with a moving wall (the fused step with the wall term),
+    measure() (the force-accumulating step) and an MP run with displacement
+    moments give bit-identical results on both paths (object force and torque
+    sums, FP64 atomics in a varying order, to 1e-13)."""
+    import sunlightlb_amd as s
+    g = load_golden("static_ns_skip.npz")
+    runs = []
+    for env in ("0", "1"):
+        monkeypatch.setenv("SLB_INDEX64", env)
+        lat = build_static(g, prec)
+        L, h = lat.L, lat.h
+        lat.s.objects_moving = 1
+        objs = lat.objects()
+        objs[-1].velx = 0.003          # the wall (inserted first, listed last)
+        objs[0].angz = 0.001
+        L.vspread(h)
+        lat.step(20)
+        L.measure(h)
+        pops = lat.pops()
+        rows = objrows(lat)
+        vmean = (lat.s.vmeanx, lat.s.vmeany, lat.s.vmeanz)
+        lat.s.objects_moving = 0
+        mp = L.newmp(h, b"null")
+        mp.contents.alpha = 0.05
+        L.mpinit(mp)
+        L.mpall(mp, 1.0)
+        L.slb_mp_track_moments(mp, 0)
+        L.slb_mp_run(mp, 12)
+        sums = (C.c_double * 6)()
+        L.slb_mp_sums(mp, sums)
+        L.slb_mp_sync_host(mp)
+        p = s.view(mp.contents.p, lat.N).copy()
+        runs.append((pops, rows, vmean, p, list(sums)))
+        lat.free()
+    a, b = runs
+    assert np.array_equal(a[0].view(np.uint64), b[0].view(np.uint64))
+    # object sums are FP64 atomics (their order varies between runs): 1e-13
+    scale = np.abs(a[1]).max()
+    assert np.abs(a[1] - b[1]).max() <= 1e-13 * scale
+    assert a[2] == b[2]
+    assert np.array_equal(a[3].view(np.uint64), b[3].view(np.uint64))
+    assert a[4] == b[4]
+    assert np.abs(a[1][:, 9:15]).max() > 0      # forces and torques were measured
