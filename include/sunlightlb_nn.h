@@ -43,6 +43,10 @@ void slbn_conv_wgrad_small(const uint16_t *X, int cin, const uint16_t *dY, int c
 void slbn_wtrans(const uint16_t *W, int cout, int cin, uint16_t *WT);
 void slbn_relu_bwd(const uint16_t *Y, const uint16_t *dY, uint16_t *dP, long n);
 void slbn_bias_grad(const uint16_t *dP, int C, int N, float *db);
+/* dP = dY * (Y > 0) (N nodes x C channels) and db[c] += sum_n dP[n][c] in one
+ * pass: slbn_relu_bwd followed by slbn_bias_grad */
+void slbn_relu_bwd_bias(const uint16_t *Y, const uint16_t *dY, uint16_t *dP, int C, int N,
+                        float *db);
 void slbn_pool(const uint16_t *X, uint16_t *Y, int C, int D, int H, int Wd);
 void slbn_pool_bwd(const uint16_t *dY, uint16_t *dX, int C, int D, int H, int Wd);
 void slbn_up_add(const uint16_t *Xc, const uint16_t *S, uint16_t *Y, int C, int D, int H, int Wd);

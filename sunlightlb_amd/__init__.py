@@ -212,6 +212,7 @@ SIGNATURES = {
     "slbn_wtrans": (None, [VP, I, I, VP]),
     "slbn_relu_bwd": (None, [VP, VP, VP, C.c_long]),
     "slbn_bias_grad": (None, [VP, I, I, VP]),
+    "slbn_relu_bwd_bias": (None, [VP, VP, VP, I, I, VP]),
     "slbn_pool": (None, [VP, VP, I, I, I, I]), "slbn_pool_bwd": (None, [VP, VP, I, I, I, I]),
     "slbn_up_add": (None, [VP, VP, VP, I, I, I, I]), "slbn_up_bwd": (None, [VP, VP, I, I, I, I]),
     "slbn_add_into": (None, [VP, VP, C.c_long]),

@@ -326,11 +326,13 @@ static void conv_bwd(slb_warmnet *net, int L, const uint16_t *gy, uint16_t *gp, 
   const long n = net->nl[LLEVEL[L]];
   const uint16_t *dp = gy;
   if (L != OUT) {
-    slbn_relu_bwd(net->act[L], gy, gp, n * LCOUT[L]);
+    /* ReLU backward and the bias gradient in one pass */
+    slbn_relu_bwd_bias(net->act[L], gy, gp, LCOUT[L], (int)n, net->g + net->bofs[L]);
     dp = gp;
+  } else {
+    slbn_bias_grad(dp, LCOUT[L], (int)n, net->g + net->bofs[L]);
   }
   slbn_conv_wgrad(layer_in(net, L), LCIN[L], dp, LCOUT[L], net->g + net->wofs[L], d[0], d[1], d[2]);
-  slbn_bias_grad(dp, LCOUT[L], (int)n, net->g + net->bofs[L]);
   if (gx)
     slbn_conv(dp, LCOUT[L], net->wt[L], NULL, gx, NULL, LCIN[L], 0, 1, d[0], d[1], d[2]);
 }

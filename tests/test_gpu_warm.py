@@ -224,6 +224,33 @@ def test_pool_upsample_and_elementwise_kernels():
         d.free()
 
 
+@pytest.mark.parametrize("C,N", [(16, 5000), (32, 3001), (64, 777)])
+def test_fused_relu_backward_and_bias_gradient(C, N):
+    """slbn_relu_bwd_bias writes the same dP as slbn_relu_bwd (bit for bit) and
+    adds the bias gradient slbn_bias_grad computes from it (FP32, float
+    atomics over blocks: to rounding), and both match the direct formula."""
+    L = lib()
+    rng = np.random.default_rng(50 + C)
+    yb, y = bf_random(rng, (N, C))
+    gb, g = bf_random(rng, (N, C))
+    dY, dG = Dev(host=yb), Dev(host=gb)
+    dP1, dP2, db1, db2 = Dev(N * C * 2), Dev(N * C * 2), Dev(C * 4), Dev(C * 4)
+    L.slbn_relu_bwd(dY.p, dG.p, dP1.p, N * C)
+    L.slbn_bias_grad(dP1.p, C, N, db1.p)
+    L.slbn_relu_bwd_bias(dY.p, dG.p, dP2.p, C, N, db2.p)
+    L.slbn_sync()
+    p1, p2 = dP1.get(np.uint16, (N, C)), dP2.get(np.uint16, (N, C))
+    b1, b2 = db1.get(np.float32, (C,)), db2.get(np.float32, (C,))
+    for d in (dY, dG, dP1, dP2, db1, db2):
+        d.free()
+    assert np.array_equal(p1, p2)
+    assert np.array_equal(p2, np.where(y > 0, gb, 0).astype(np.uint16))
+    scale = np.abs(g).sum(0).max()
+    assert np.abs(b2 - b1).max() <= 1e-6 * scale
+    ref = np.where(y > 0, g, 0.0).sum(0)
+    assert np.abs(b2 - ref).max() < 1e-5 * scale
+
+
 def test_loss_and_adam_kernels():
     L = lib()
     rng = np.random.default_rng(41)
