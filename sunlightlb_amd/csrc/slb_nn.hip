@@ -1112,9 +1112,8 @@ static void wgrad_staged(const uint16_t *X, int cin, const uint16_t *dY, int cou
 
 /* the LDS-staged 16 -> 16 weight gradient applies (SLBN_WGRAD_LDS=0 turns it off) */
 extern "C" int slbn_wgrad_lds16_ok(int cin, int cout, int D, int H, int Wd) {
-  static int env = -1;
-  if (env < 0) env = getenv("SLBN_WGRAD_LDS") ? atoi(getenv("SLBN_WGRAD_LDS")) : 1;
-  return env != 0 && cin == 16 && cout == 16 && D >= 1 && H >= 2 && H % 2 == 0 &&
+  const char *env = getenv("SLBN_WGRAD_LDS");   /* read per call, as SLBN_CONV_LDS */
+  return !(env && atoi(env) == 0) && cin == 16 && cout == 16 && D >= 1 && H >= 2 && H % 2 == 0 &&
          Wd >= 8 && Wd % 8 == 0 && Wd <= 128;
 }
 

@@ -143,6 +143,7 @@ def test_conv_input_gradient_matches_oracle(shape, cin, cout):
                                                   ((8, 16, 16), 32, 16, 0),
                                                   ((6, 10, 24), 16, 16, 0), ((3, 2, 8), 16, 16, 0),
                                                   ((5, 6, 128), 16, 16, 0), ((4, 8, 40), 16, 16, 0),
+                                                  ((4, 5, 16), 16, 16, 0), ((2, 4, 136), 16, 16, 0),
                                                   ((8, 8, 12), 8, 16, 1), ((6, 8, 10), 16, 8, 1)])
 def test_conv_weight_gradient_matches_oracle(shape, cin, cout, small):
     L = lib()
