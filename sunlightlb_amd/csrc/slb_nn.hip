@@ -163,8 +163,8 @@ k_conv(const uint16_t *__restrict__ X, int Cin, const uint16_t *__restrict__ W, 
 /* The same convolution for CIN = 16, 32 or 64 input channels with the input
  * brick staged in LDS: a block computes XT = 1024/CIN x-nodes x 4 rows of
  * ZT planes (wave w: row w, XT/16 tiles of 16 nodes per plane) and first
- * loads the (XT+2) x 6 x (ZT+2) halo brick (38-41 KB for one plane, 51 KB for
- * two; periodic wrap applied at load) once,
+ * loads the (XT+2) x 6 x (ZT+2) halo brick (38-41 KB for one plane, 51-55 KB
+ * for two; periodic wrap applied at load) once,
  * so the 27 taps read the input from LDS instead of 27 times through L2.
  * The K loop, MFMA order and epilogue are those of k_conv: results are
  * bit-identical.  Needs Wd % XT == 0 and H % 4 == 0. */
@@ -1043,20 +1043,26 @@ extern "C" void slbn_conv(const uint16_t *X, int cin, const uint16_t *W, const f
   const int xt = cin >= 16 ? 1024 / cin : 0;
   if ((cin == 16 || cin == 32 || cin == 64) && Wd % xt == 0 && H % 4 == 0 && mt <= 4 &&
       !(lds_env && atoi(lds_env) == 0)) {
-    /* two planes per block for the thin layers (cin 16, at most 32 outputs) */
-    const int zt = (cin == 16 && mt <= 2 && D % 2 == 0) ? 2 : 1;
+    /* two planes per block where D is even and it measured faster (128^3
+     * network, us per call): 16 inputs with up to 32 outputs (96.6 -> 83.4),
+     * 32 -> 32 (43.0 -> 41.0), 64 inputs (41.6 -> 38.8, 32.2 -> 28.3); not
+     * 32 -> 16 (32.1 -> 32.9) or 32 -> 64 (24.7 -> 30.7) */
+    const int zt = (D % 2 == 0 && ((cin == 16 && mt <= 2) || (cin == 32 && mt == 2) ||
+                                   cin == 64)) ? 2 : 1;
     const dim3 gl((unsigned)((Wd / xt) * (H / 4) * (D / zt)));
 #define SLBN_CL(C, MT, ZT)                                                                  \
   hipLaunchKernelGGL((k_conv_lds<C, MT, ZT>), gl, dim3(256), 0, nn_stream, X, W, Kp, bias, Y, \
                      Yf, cout, relu, sgn, G)
-#define SLBN_CLM(C)                                                                         \
-  if (mt == 1) SLBN_CL(C, 1, 1); else if (mt == 2) SLBN_CL(C, 2, 1);                        \
-  else if (mt == 3) SLBN_CL(C, 3, 1); else SLBN_CL(C, 4, 1);
+#define SLBN_CLM(C, ZT)                                                                     \
+  if (mt == 1) SLBN_CL(C, 1, ZT); else if (mt == 2) SLBN_CL(C, 2, ZT);                      \
+  else if (mt == 3) SLBN_CL(C, 3, ZT); else SLBN_CL(C, 4, ZT);
     if (cin == 16 && zt == 2) {
       if (mt == 1) SLBN_CL(16, 1, 2); else SLBN_CL(16, 2, 2);
-    } else if (cin == 16) { SLBN_CLM(16) }
-    else if (cin == 32) { SLBN_CLM(32) }
-    else { SLBN_CLM(64) }
+    } else if (cin == 16) { SLBN_CLM(16, 1) }
+    else if (cin == 32 && zt == 2) { SLBN_CL(32, 2, 2); }
+    else if (cin == 32) { SLBN_CLM(32, 1) }
+    else if (zt == 2) { SLBN_CLM(64, 2) }
+    else { SLBN_CLM(64, 1) }
 #undef SLBN_CLM
 #undef SLBN_CL
     LAUNCHCHECK();

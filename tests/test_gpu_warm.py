@@ -544,11 +544,13 @@ def test_start_policy_weights_file_and_cold_start():
     ((4, 8, 64), 16, 16, 0), ((3, 4, 128), 16, 32, 1), ((4, 8, 64), 16, 48, 0),
     ((2, 4, 64), 16, 64, 1), ((4, 8, 32), 32, 32, 0), ((3, 4, 64), 32, 64, 1),
     ((4, 4, 16), 64, 64, 0), ((2, 8, 32), 64, 16, 1),
-    ((2, 4, 128), 16, 32, 1), ((6, 4, 64), 16, 16, 1), ((8, 8, 64), 16, 24, 0)])
+    ((2, 4, 128), 16, 32, 1), ((6, 4, 64), 16, 16, 1), ((8, 8, 64), 16, 24, 0),
+    ((2, 4, 32), 32, 16, 0), ((2, 4, 32), 32, 64, 1)])
 def test_conv_lds_kernel_is_bit_identical_and_matches_oracle(shape, cin, cout, transposed):
     """16, 32 or 64 input channels on rows of a multiple of 1024/cin nodes take
-    the LDS-staged kernel (two planes per block for 16 inputs, at most 32
-    outputs and an even z extent; one plane otherwise); SLBN_CONV_LDS=0
+    the LDS-staged kernel (two planes per block for an even z extent and 16
+    inputs with at most 32 outputs, 32 -> 32, or 64 inputs; one plane
+    otherwise); SLBN_CONV_LDS=0
     forces the general one: outputs
     bit-identical (same K order and MFMA sequence), and within FP32 rounding
     of the oracle."""
