@@ -19,6 +19,7 @@ struct slb_batch_s {
 
 SLBBatch slb_batch_new(int prec) {
   SLBBatch b = (SLBBatch)calloc(1, sizeof(struct slb_batch_s));
+  if (b == NULL) error("slb_batch_new: out of host memory");
   b->prec = (prec == 32) ? 32 : 64;
   b->d = slbd_batch_create(b->prec);
   return b;
@@ -43,7 +44,7 @@ int slb_batch_add(SLBBatch b, LBSys lb, int nrhs, const double *objvel) {
     }
   }
   const int nobj = S->nobj;
-  double *tabs = (double *)malloc(sizeof(double) * SLBD_OBJ_STRIDE * (nobj > 0 ? nobj : 1) * nrhs);
+  double *tabs = (double *)slb_xmalloc(sizeof(double) * SLBD_OBJ_STRIDE * (nobj > 0 ? nobj : 1) * nrhs);
   for (int r = 0; r < nrhs; r++)
     for (int o = 0; o < nobj; o++) {
       double *t = tabs + ((long)r * nobj + o) * SLBD_OBJ_STRIDE;

@@ -91,7 +91,7 @@ LBSys newlb(int reqesi, int reqesj, int reqesk, char *name) {
   S->prec = default_precision();
   S->dev = slbd_create(reqesi, reqesj, reqesk, S->prec);
   S->pops_loc = S->u_loc = S->f_loc = S->mask_loc = S->rho_loc = LOC_SYNC;
-  S->objidx = (int32_t *)malloc(sizeof(int32_t) * (size_t)(N > 0 ? N : 1));
+  S->objidx = (int32_t *)slb_xmalloc(sizeof(int32_t) * (size_t)(N > 0 ? N : 1));
   for (long n = 0; n < N; n++) S->objidx[n] = -1;
   S->objidx_dirty = 0;
   S->bf_dirty = 0;
@@ -215,8 +215,8 @@ static void build_objidx(LBSys lb) {
   slb_objtab(lb);
   int cap = 16;
   while (cap < 2 * S->nobj + 2) cap *= 2;
-  LBObj *keys = (LBObj *)calloc(cap, sizeof(LBObj));
-  int *vals = (int *)malloc(sizeof(int) * cap);
+  LBObj *keys = (LBObj *)slb_xcalloc(cap, sizeof(LBObj));
+  int *vals = (int *)slb_xmalloc(sizeof(int) * cap);
   for (int o = 0; o < S->nobj; o++) {
     uint64_t h = ptr_hash(S->objlist[o]) & (cap - 1);
     while (keys[h] != NULL) h = (h + 1) & (cap - 1);

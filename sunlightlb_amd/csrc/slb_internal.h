@@ -15,6 +15,28 @@
 #include "sunlightlb_gpu.h"
 #include "slb_dev.h"
 
+/* error() prints and exits (src/lbmisc.c:39-42); said here so that the
+ * compiler and its analyser know that no call returns */
+void error(char *s1) __attribute__((noreturn));
+
+/* host allocations that end the program with error() when memory runs out */
+#include <stdlib.h>
+static inline void *slb_xmalloc(size_t n) {
+  void *p = malloc(n ? n : 1);
+  if (p == NULL) error("out of host memory");
+  return p;
+}
+static inline void *slb_xcalloc(size_t n, size_t s) {
+  void *p = calloc(n ? n : 1, s ? s : 1);
+  if (p == NULL) error("out of host memory");
+  return p;
+}
+static inline void *slb_xrealloc(void *q, size_t n) {
+  void *p = realloc(q, n ? n : 1);
+  if (p == NULL) error("out of host memory");
+  return p;
+}
+
 /* Where the newest copy of a mirrored array lives. */
 enum { LOC_SYNC = 0, LOC_HOST = 1, LOC_DEV = 2 };
 

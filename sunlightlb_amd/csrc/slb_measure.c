@@ -133,8 +133,8 @@ void distribf(LBSys lb) {
   slb_dev_f(lb);
   slb_objtab(lb);
   if (S->nobj > 0) {
-    double *frc = (double *)malloc(sizeof(double) * 3 * S->nobj);
-    double *trq = (double *)malloc(sizeof(double) * 3 * S->nobj);
+    double *frc = (double *)slb_xmalloc(sizeof(double) * 3 * S->nobj);
+    double *trq = (double *)slb_xmalloc(sizeof(double) * 3 * S->nobj);
     slbd_distribf(S->dev, S->nobj, S->objtab, frc, trq);
     for (int o = 0; o < S->nobj; o++) {
       LBObj p = S->objlist[o];

@@ -217,7 +217,7 @@ void wrmon(LBSys lb, char *what) {
 }
 
 char *mycat(char *s, char *t) {
-  char *v = (char *)malloc(strlen(s) + strlen(t) + 1);
+  char *v = (char *)slb_xmalloc(strlen(s) + strlen(t) + 1);
   strcpy(v, s);
   strcat(v, t);
   return v;

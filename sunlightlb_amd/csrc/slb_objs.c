@@ -469,6 +469,7 @@ void wrmap(LBSys lb, char *name) {
     return;
   }
   int *count = (int *)calloc(ng, sizeof(int));
+  if (count == NULL) error("wrmap: out of host memory");
   for (int k = 0; k < lb->esk; k++) {
     fprintf(fp, "\nSlice k = %3i\n", k);
     fprintf(fp, "            ");
@@ -559,8 +560,8 @@ static int rsa_overlaps(const rsa_grid *g, const double *c, double d2min) {
 static void rsa_add(rsa_grid *g, const double *c) {
   if (g->n == g->cap) {
     g->cap = g->cap ? 2 * g->cap : 1024;
-    g->xyz = (double *)realloc(g->xyz, sizeof(double) * 3 * g->cap);
-    g->next = (int *)realloc(g->next, sizeof(int) * g->cap);
+    g->xyz = (double *)slb_xrealloc(g->xyz, sizeof(double) * 3 * g->cap);
+    g->next = (int *)slb_xrealloc(g->next, sizeof(int) * g->cap);
   }
   int ci[3];
   for (int d = 0; d < 3; d++) {
@@ -588,7 +589,7 @@ int slb_rsa_pack(LBSys lb, double R, double phi, unsigned long long seed, long m
     g.cs[d] = g.L[d] / g.nc[d];
     ncell *= g.nc[d];
   }
-  g.head = (int *)malloc(sizeof(int) * ncell);
+  g.head = (int *)slb_xmalloc(sizeof(int) * ncell);
   for (int c = 0; c < ncell; c++) g.head[c] = -1;
   uint64_t st = (uint64_t)seed;
   long target = (long)ceil(phi * (double)lb->nnodes);

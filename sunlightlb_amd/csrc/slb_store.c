@@ -274,13 +274,15 @@ void lbsavestr(LBSys lb, char *savefile, char *type) {
 
 /* Object state as text records POS/VEL/ANG/FRC/TRQ (src/lbstore.c:523-546). */
 void lbsaveobj(LBSys lb, char *savefile) {
-  int more = 0;
+  int more = 0, opened = 0;
   FILE *fp;
   if (myscmp("stdout", savefile)) { fp = stdout; more = 1; }
   else if (myscmp("stderr", savefile)) { fp = stderr; more = 1; }
   else if ((fp = fopen(savefile, "w")) == NULL) {
     printf("lbsaveobj: %s could not be opened\n", savefile);
     return;
+  } else {
+    opened = 1;
   }
   printf("\nSaving object data to %s\n", savefile);
   if (more) fprintf(fp, "START of particle data: TYPE, id, base id, x, y, z\n\n");
@@ -294,7 +296,7 @@ void lbsaveobj(LBSys lb, char *savefile) {
     if (more) fprintf(fp, "\n");
   }
   if (more) fprintf(fp, "END of particle data\n");
-  if (fp != stdout && fp != stderr) fclose(fp);
+  if (opened) fclose(fp);
   if (slb_verbosity & TRACE_ROUTINE_EXITS) printf("LB %i : lbsaveobj done\n", lb->step);
 }
 
