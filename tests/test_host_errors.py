@@ -43,3 +43,25 @@ def test_newlb_rejects_bad_dimensions():
         assert p.returncode == 1, (dims, p.stdout, p.stderr)
         assert "lattice dimensions" in p.stderr
         assert "returned" not in p.stdout
+
+
+def test_string_helpers_need_no_device():
+    """mycat, mycopy and myscmp (src/lbmisc.c) are host code: concatenation and
+    copy into fresh memory, and myscmp(s, t) true when t starts with s (the
+    file-name tests "null", "stdout", "stderr" of the writers rely on it)."""
+    import ctypes as C
+    L = lib()
+    libc = C.CDLL(None)
+    libc.free.argtypes = [C.c_void_p]
+    p = L.mycat(b"sphere", b".dat")
+    q = L.mycopy(b"null")
+    try:
+        assert C.string_at(p) == b"sphere.dat"
+        assert C.string_at(q) == b"null"
+    finally:
+        libc.free(p)
+        libc.free(q)
+    assert L.myscmp(b"null", b"null") == 1
+    assert L.myscmp(b"std", b"stdout") == 1
+    assert L.myscmp(b"stdout", b"std") == 0
+    assert L.myscmp(b"null", b"nul") == 0

@@ -330,7 +330,8 @@ def _sha(a):
     return hashlib.sha256(np.ascontiguousarray(a, dtype="<i4").tobytes()).hexdigest()
 
 
-@pytest.mark.parametrize("name", ["pack128", "pack_noncubic", "box0", "box3"])
+@pytest.mark.parametrize("name", ["pack128", "pack_noncubic", "box0", "box1", "box2", "box3",
+                                  "box4", "box5"])
 def test_tables_at_workload_sizes(oracle, name):
     """Solid map and boundary-link mask of workload-size packs (128^3 radius-6
     pack, a non-cubic mixed-radius pack, batch-like boxes) equal the
