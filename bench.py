@@ -339,12 +339,14 @@ def insert_periodic(L, h, Lb, R, c):
     return objs
 
 
-def batch_run(L, args, prec, boxes, u0, tau, tune):
-    """One batched run in the given precision: returns the throughput record
-    and the 6x6 resistance matrix of sphere 0 in every box."""
+def build_batch(L, prec, boxes, u0, tau):
+    """The batch of the resistance workload: every box with its spheres
+    (periodic images as separate objects, sphere 0 inserted last so that
+    its images head the object list) and 6 right-hand sides -- sphere 0
+    translating along x, y, z at u0, then rotating about x, y, z at u0/R0 --
+    the other spheres fixed.  Returns the uncommitted batch and, per box,
+    (object rows of sphere 0's images, object count, R0)."""
     import numpy as np
-    os.environ["SLB_PRECISION"] = str(prec)
-    t0 = time.perf_counter()
     b = L.slb_batch_new(prec)
     moving_rows = []
     for (Lb, sph) in boxes:
@@ -362,15 +364,50 @@ def batch_run(L, args, prec, boxes, u0, tau, tune):
             order.append(C.addressof(p.contents))
             p = p.contents.next_obj
         rows = [order.index(a) for a in mover]
-        vel = np.zeros((6, len(order), 6))
-        R0 = sph[0][0]
-        for k in range(6):
-            amp = u0 if k < 3 else u0 / R0
-            vel[k, rows, k] = amp
+        vel = rhs_velocities(len(order), rows, sph[0][0], u0)
         flat = np.ascontiguousarray(vel.reshape(-1))
         L.slb_batch_add(b, h, 6, flat.ctypes.data_as(C.POINTER(C.c_double)))
-        moving_rows.append((rows, len(order), R0))
+        moving_rows.append((rows, len(order), sph[0][0]))
         L.slb_destroy(h)
+    return b, moving_rows
+
+
+def rhs_velocities(nobj, rows, R0, u0):
+    """(6, nobj, 6) object velocity rows of the 6 right-hand sides."""
+    import numpy as np
+    vel = np.zeros((6, nobj, 6))
+    for k in range(6):
+        vel[k, rows, k] = u0 if k < 3 else u0 / R0
+    return vel
+
+
+def resistance_matrices(L, b, moving_rows, u0):
+    """6x6 resistance matrix of sphere 0 in every box after
+    slb_batch_measure: column k = (force, torque) summed over sphere 0's
+    images for right-hand side k, per unit velocity."""
+    import numpy as np
+    mats = []
+    for q, (rows, nobj, R0) in enumerate(moving_rows):
+        Rm = np.zeros((6, 6))
+        for k in range(6):
+            frc = np.zeros(3 * nobj)
+            trq = np.zeros(3 * nobj)
+            L.slb_batch_forces(b, q, k, frc.ctypes.data_as(C.POINTER(C.c_double)),
+                               trq.ctypes.data_as(C.POINTER(C.c_double)))
+            F = frc.reshape(nobj, 3)[rows].sum(0)
+            T = trq.reshape(nobj, 3)[rows].sum(0)
+            amp = u0 if k < 3 else u0 / R0
+            Rm[:, k] = np.concatenate([F, T]) / amp
+        mats.append(Rm)
+    return mats
+
+
+def batch_run(L, args, prec, boxes, u0, tau, tune):
+    """One batched run in the given precision: returns the throughput record
+    and the 6x6 resistance matrix of sphere 0 in every box."""
+    os.environ["SLB_PRECISION"] = str(prec)
+    t0 = time.perf_counter()
+    b, moving_rows = build_batch(L, prec, boxes, u0, tau)
     L.slb_batch_commit(b)
     setup = time.perf_counter() - t0
     t1 = time.perf_counter()
@@ -385,19 +422,7 @@ def batch_run(L, args, prec, boxes, u0, tau, tune):
     L.slb_batch_measure(b)
     mlups = nodes * args.batch_steps / (ms * 1e-3) / 1e6
     bpl = 2 * 15 * (4 if prec == 32 else 8)
-    mats = []
-    for q, (rows, nobj, R0) in enumerate(moving_rows):
-        Rm = np.zeros((6, 6))
-        for k in range(6):
-            frc = np.zeros(3 * nobj)
-            trq = np.zeros(3 * nobj)
-            L.slb_batch_forces(b, q, k, frc.ctypes.data_as(C.POINTER(C.c_double)),
-                               trq.ctypes.data_as(C.POINTER(C.c_double)))
-            F = frc.reshape(nobj, 3)[rows].sum(0)
-            T = trq.reshape(nobj, 3)[rows].sum(0)
-            amp = u0 if k < 3 else u0 / R0
-            Rm[:, k] = np.concatenate([F, T]) / amp
-        mats.append(Rm)
+    mats = resistance_matrices(L, b, moving_rows, u0)
     L.slb_batch_free(b)
     res = {
         "workload": f"{len(boxes)} periodic boxes, edge U{{32..64}}, 2-20 spheres radius "

@@ -272,6 +272,14 @@ class Lattice:
             if L.slb_set_precision(self.h, int(precision)) != 0:
                 raise ValueError("precision must be 32 or 64")
 
+    @classmethod
+    def wrap(cls, h):
+        """A wrapper around an existing LBSys handle (not owned: free()
+        destroys it)."""
+        self = cls.__new__(cls)
+        self.L, self.h, self.s = lib(), h, h.contents
+        return self
+
     @property
     def shape(self):
         return (self.s.esi, self.s.esj, self.s.esk)

@@ -218,10 +218,22 @@ struct slbd_batch {
   int block, npt, grid;
   int tiled_block, tiled_npt;   /* tile table built for this shape */
   hipStream_t stream;
-  hipGraphExec_t gexec;
-  int graph_steps, graph_valid;
+  /* captured step loops, one per live buffer: a graph replays the pointers
+   * it was captured with, so the loop captured from buffer 0 must not run
+   * when buffer 1 is live (after an odd number of plain steps) */
+  hipGraphExec_t gexec[2];
+  int graph_valid[2];
   hipEvent_t ev0, ev1;
 };
+
+/* Drop the captured step loops (launch shape or tile table changed). */
+static void invalidate_graphs(slbd_batch *B) {
+  for (int c = 0; c < 2; c++)
+    if (B->graph_valid[c]) {
+      HIPCHECK(hipGraphExecDestroy(B->gexec[c]));
+      B->graph_valid[c] = 0;
+    }
+}
 
 static size_t bts(int prec) { return prec == 32 ? 4 : 8; }
 
@@ -284,7 +296,7 @@ static void build_tiles(slbd_batch *B) {
   B->ntiles = (int)nt;
   B->tiled_block = B->block;
   B->tiled_npt = B->npt;
-  B->graph_valid = 0;
+  invalidate_graphs(B);
 }
 
 extern "C" int slbd_batch_commit(slbd_batch *B) {
@@ -414,39 +426,43 @@ static void bstep(slbd_batch *B, bool force) {
 extern "C" void slbd_batch_config(slbd_batch *B, int block, int npt, int grid) {
   if (block != 64 && block != 128 && block != 256 && block != 512) block = 256;
   if (npt != 1 && npt != 2 && npt != 4) npt = 1;
-  if (block != B->block || npt != B->npt || grid != B->grid) B->graph_valid = 0;
+  if (block != B->block || npt != B->npt || grid != B->grid) invalidate_graphs(B);
   B->block = block; B->npt = npt; B->grid = grid;
 }
 
 #define BATCH_GRAPH_STEPS 32
 
+/* Capture BATCH_GRAPH_STEPS steps starting from the live buffer B->cur. */
 static void build_graph(slbd_batch *B) {
   if (B->tiled_block != B->block || B->tiled_npt != B->npt) build_tiles(B);
-  if (B->graph_valid) HIPCHECK(hipGraphExecDestroy(B->gexec));
-  hipGraph_t g;
   const int cur0 = B->cur;
+  if (B->graph_valid[cur0]) return;
+  hipGraph_t g;
   HIPCHECK(hipStreamBeginCapture(B->stream, hipStreamCaptureModeThreadLocal));
   for (int s = 0; s < BATCH_GRAPH_STEPS; s++) bstep(B, false);
   HIPCHECK(hipStreamEndCapture(B->stream, &g));
-  HIPCHECK(hipGraphInstantiate(&B->gexec, g, NULL, NULL, 0));
+  HIPCHECK(hipGraphInstantiate(&B->gexec[cur0], g, NULL, NULL, 0));
   HIPCHECK(hipGraphDestroy(g));
   B->cur = cur0;   /* an even number of steps returns to the same buffer */
-  B->graph_valid = 1;
+  B->graph_valid[cur0] = 1;
 }
 
 extern "C" void slbd_batch_run(slbd_batch *B, int nsteps, int use_graph) {
   if (!B->committed) return;
   int s = 0;
   if (use_graph && nsteps >= BATCH_GRAPH_STEPS) {
-    if (!B->graph_valid) build_graph(B);
+    /* the loop captured from the live buffer (a shape change since the
+     * last capture rebuilds the tile table and drops both loops) */
+    if (B->tiled_block != B->block || B->tiled_npt != B->npt) build_tiles(B);
+    build_graph(B);
     for (; s + BATCH_GRAPH_STEPS <= nsteps; s += BATCH_GRAPH_STEPS)
-      HIPCHECK(hipGraphLaunch(B->gexec, B->stream));
+      HIPCHECK(hipGraphLaunch(B->gexec[B->cur], B->stream));
   }
   for (; s < nsteps; s++) bstep(B, false);
 }
 
 extern "C" double slbd_batch_time(slbd_batch *B, int nsteps, int use_graph) {
-  if (use_graph && !B->graph_valid) build_graph(B);
+  if (use_graph && B->committed && nsteps >= BATCH_GRAPH_STEPS) build_graph(B);
   HIPCHECK(hipStreamSynchronize(B->stream));
   HIPCHECK(hipEventRecord(B->ev0, B->stream));
   slbd_batch_run(B, nsteps, use_graph);
@@ -527,7 +543,7 @@ extern "C" int slbd_batch_cu_count(void) {
 extern "C" void slbd_batch_destroy(slbd_batch *B) {
   if (!B) return;
   HIPCHECK(hipStreamSynchronize(B->stream));
-  if (B->graph_valid) HIPCHECK(hipGraphExecDestroy(B->gexec));
+  invalidate_graphs(B);
   void *ps[] = {B->pops[0], B->pops[1], B->flags, B->objidx, B->tab, B->wtab, B->acc, B->probs,
                 B->tiles};
   for (size_t q = 0; q < sizeof ps / sizeof ps[0]; q++)

@@ -71,6 +71,13 @@ void slbd_scale_pops(slbd_lattice *L, double s);
 /* LB step: nsteps fused bbl+propagate+collide; force != 0 accumulates the
  * bblforce node force (fx,fy,fz) during the bounce-back. */
 void slbd_step(slbd_lattice *L, const slbd_params *P, int nsteps, int force);
+/* Launch plan of the two-step sweep over esi planes in chunks of P (see
+ * sweep2 in slb_dev.hip): rows of {phase, i0, np, sring, dring}, phase 1 =
+ * step t from the live buffer into the plane window of the other buffer,
+ * phase 2 = step t+1 from the window back into the live buffer.  Returns
+ * the launch count (rows written up to maxrows), 0 when the sweep does not
+ * apply.  Host-only. */
+int slbd_sweep_plan(int esi, int P, int *rows, int maxrows);
 /* Separate phases on the live buffer. */
 void slbd_bbl(slbd_lattice *L, const slbd_params *P, int force);
 void slbd_propagate(slbd_lattice *L);

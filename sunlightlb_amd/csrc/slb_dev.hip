@@ -41,6 +41,22 @@
 /* ---------------------------------------------------------------------- */
 /* Fused step kernel.                                                      */
 
+/* Plane addressing of one step launch.  A launch updates the planes
+ * i0 .. i0 + gridDim.y - 1 (mod esi).  Source and destination populations
+ * of plane i live at plane slot sring/dring(i) of their buffer: ring 0 is
+ * the plain layout (slot i); ring R > 0 keeps planes 0 and 1 in slots 0
+ * and 1 and plane i >= 2 in slot 2 + (i - 2) mod R, so a buffer of R + 2
+ * planes holds a sliding window of the lattice (the intermediate step of
+ * the two-step sweep).  Flags, wall velocities, body and node forces are
+ * always indexed by the global node. */
+struct SweepMap {
+  int i0, sring, dring;
+};
+
+__device__ __forceinline__ int plane_slot(int i, int ring) {
+  return (ring == 0 || i < 2) ? i : 2 + (i - 2) % ring;
+}
+
 /* I: index type, int when the lattice holds fewer than 2^31 populations
  * (half the address arithmetic), long otherwise. */
 template <typename T, bool MOVING, bool FORCE, int BF, typename I>
@@ -51,35 +67,47 @@ k_step(const T *__restrict__ src, T *__restrict__ dst,
        const double *__restrict__ uz,
        const T *__restrict__ bx, const T *__restrict__ by, const T *__restrict__ bz,
        double *__restrict__ fx, double *__restrict__ fy, double *__restrict__ fz,
-       Geom G, PhysArgs P) {
+       Geom G, PhysArgs P, SweepMap M) {
   int blk_x, blk_y;
   step_block(G, blk_x, blk_y);
   const int t = blk_x * blockDim.x + threadIdx.x;
   if (t >= G.plane) return;
-  const int i = blk_y;
+  int i = blk_y + M.i0;
+  if (i >= G.esi) i -= G.esi;
   const int j = t / G.esk;
   const int k = t - j * G.esk;
   const I x = (I)i * (I)G.plane + (I)t;
   const AxisOffT<I> oi = axis_off_t<I>(i, G.esi, (I)G.plane);
   const AxisOffT<I> oj = axis_off_t<I>(j, G.esj, (I)G.esk);
   const AxisOffT<I> ok = axis_off_t<I>(k, G.esk, (I)1);
+  /* source node and plane offsets in the source buffer's slots */
+  I xs = x;
+  AxisOffT<I> os = oi;
+  if (M.sring) {
+    const int si = plane_slot(i, M.sring);
+    const int im = i > 0 ? i - 1 : G.esi - 1, ip = i < G.esi - 1 ? i + 1 : 0;
+    xs = (I)si * (I)G.plane + (I)t;
+    os.m = (I)(plane_slot(im, M.sring) - si) * (I)G.plane;
+    os.p = (I)(plane_slot(ip, M.sring) - si) * (I)G.plane;
+  }
+  const I xd = M.dring ? (I)plane_slot(i, M.dring) * (I)G.plane + (I)t : x;
   const unsigned fl = flags[x];
   const bool solid = (fl & FLAG_SOLID) != 0;
   T v[15];
   if (G.nt_load) {
-    v[0] = __builtin_nontemporal_load(&src[pidx_t<I>(0, x)]);
+    v[0] = __builtin_nontemporal_load(&src[pidx_t<I>(0, xs)]);
     Links<1, 15>::run([&](auto LL) {
       constexpr int l = decltype(LL)::value;
       const bool bb = (fl >> (l - 1)) & 1u;
-      const I idx = bb ? pidx_t<I>(OPP(l), x) : pidx_t<I>(l, x + nb_off<l, -1>(oi, oj, ok));
+      const I idx = bb ? pidx_t<I>(OPP(l), xs) : pidx_t<I>(l, xs + nb_off<l, -1>(os, oj, ok));
       v[l] = __builtin_nontemporal_load(&src[idx]);
     });
   } else {
-    v[0] = src[pidx_t<I>(0, x)];
+    v[0] = src[pidx_t<I>(0, xs)];
     Links<1, 15>::run([&](auto LL) {
       constexpr int l = decltype(LL)::value;
       const bool bb = (fl >> (l - 1)) & 1u;
-      const I idx = bb ? pidx_t<I>(OPP(l), x) : pidx_t<I>(l, x + nb_off<l, -1>(oi, oj, ok));
+      const I idx = bb ? pidx_t<I>(OPP(l), xs) : pidx_t<I>(l, xs + nb_off<l, -1>(os, oj, ok));
       v[l] = src[idx];
     });
   }
@@ -88,17 +116,16 @@ k_step(const T *__restrict__ src, T *__restrict__ dst,
     Links<1, 15>::run([&](auto LL) {
       constexpr int l = decltype(LL)::value;
       if ((fl >> (l - 1)) & 1u) {
-        const I xs = x + nb_off<l, -1>(oi, oj, ok);
         double cu = 0.0;
         if (MOVING) {
-          const I s = solid ? x : xs;
+          const I s = solid ? x : x + nb_off<l, -1>(oi, oj, ok);
           cu = (double)WL(l) * ((CX(l) ? CX(l) * ux[s] : 0.0) + (CY(l) ? CY(l) * uy[s] : 0.0) +
                                 (CZ(l) ? CZ(l) * uz[s] : 0.0));
         }
         if (FORCE && solid) {
           /* link m = lbar of solid x: a = a_m(x), b = a_l(x - c_l) */
           const double a = (double)v[l];
-          const double b = (double)src[pidx_t<I>(l, xs)];
+          const double b = (double)src[pidx_t<I>(l, xs + nb_off<l, -1>(os, oj, ok))];
           const double g = -2.0 * (a - b + cu);
           if (CX(l)) Fx += CX(l) * g;
           if (CY(l)) Fy += CY(l) * g;
@@ -121,10 +148,10 @@ k_step(const T *__restrict__ src, T *__restrict__ dst,
   }
   if (G.nt_store) {
 #pragma unroll
-    for (int l = 0; l < 15; l++) __builtin_nontemporal_store(v[l], &dst[pidx_t<I>(l, x)]);
+    for (int l = 0; l < 15; l++) __builtin_nontemporal_store(v[l], &dst[pidx_t<I>(l, xd)]);
   } else {
 #pragma unroll
-    for (int l = 0; l < 15; l++) dst[pidx_t<I>(l, x)] = v[l];
+    for (int l = 0; l < 15; l++) dst[pidx_t<I>(l, xd)] = v[l];
   }
 }
 
@@ -758,6 +785,7 @@ struct slbd_lattice {
   void *rho;
   uint32_t *mpf;
   hipEvent_t ev0, ev1;
+  int sweep_nt_mid;         /* two-step sweep: non-temporal window stores */
 };
 
 struct slbd_mp {
@@ -835,6 +863,7 @@ extern "C" slbd_lattice *slbd_create(int esi, int esj, int esk, int prec) {
     if (b == 256 || b == 512) L->G.step_tpb = b;
   }
   L->G.xcd_map = step_order(L->G);
+  L->sweep_nt_mid = getenv("SLB_SWEEP_NT") ? atoi(getenv("SLB_SWEEP_NT")) : 0;
   {
     /* MP kernel: 256-thread blocks over (row block, plane) */
     const long nb = (long)((L->G.plane + 255) / 256) * esi;
@@ -1326,27 +1355,34 @@ static int bf_mode(const slbd_lattice *L, const slbd_params *P) {
   return 0;
 }
 
+/* One launch of the step kernel over planes i0 .. i0 + np - 1 (mod esi). */
 template <typename T, bool MOVING, bool FORCE, int BF>
-static void launch_step_t(slbd_lattice *L, const PhysArgs &A) {
-  const T *src = (const T *)L->pops[L->cur];
-  T *dst = (T *)L->pops[L->cur ^ 1];
-  const dim3 grid((unsigned)((L->G.plane + L->G.step_tpb - 1) / L->G.step_tpb), (unsigned)L->esi);
+static void launch_step_t(slbd_lattice *L, const PhysArgs &A, const void *srcv, void *dstv,
+                          int i0, int np, int sring, int dring, const Geom &G) {
+  const T *src = (const T *)srcv;
+  T *dst = (T *)dstv;
+  const SweepMap M = {i0, sring, dring};
+  const dim3 grid((unsigned)((G.plane + G.step_tpb - 1) / G.step_tpb), (unsigned)np);
   if (L->idx32)
-    hipLaunchKernelGGL((k_step<T, MOVING, FORCE, BF, int>), grid, dim3(L->G.step_tpb), 0,
+    hipLaunchKernelGGL((k_step<T, MOVING, FORCE, BF, int>), grid, dim3(G.step_tpb), 0,
                        L->stream, src, dst, L->flags, L->ux, L->uy, L->uz, (const T *)L->bx,
-                       (const T *)L->by, (const T *)L->bz, L->fx, L->fy, L->fz, L->G, A);
+                       (const T *)L->by, (const T *)L->bz, L->fx, L->fy, L->fz, G, A, M);
   else
-    hipLaunchKernelGGL((k_step<T, MOVING, FORCE, BF, long>), grid, dim3(L->G.step_tpb), 0,
+    hipLaunchKernelGGL((k_step<T, MOVING, FORCE, BF, long>), grid, dim3(G.step_tpb), 0,
                        L->stream, src, dst, L->flags, L->ux, L->uy, L->uz, (const T *)L->bx,
-                       (const T *)L->by, (const T *)L->bz, L->fx, L->fy, L->fz, L->G, A);
+                       (const T *)L->by, (const T *)L->bz, L->fx, L->fy, L->fz, G, A, M);
   LAUNCHCHECK();
-  L->cur ^= 1;
 }
 
 template <typename T>
-static void launch_step(slbd_lattice *L, const PhysArgs &A, bool moving, bool force, int bf) {
-#define SLB_STEP_CASE(M, F, B)                                      \
-  if (moving == M && force == F && bf == B) { launch_step_t<T, M, F, B>(L, A); return; }
+static void launch_step(slbd_lattice *L, const PhysArgs &A, bool moving, bool force, int bf,
+                        const void *src, void *dst, int i0, int np, int sring, int dring,
+                        const Geom &G) {
+#define SLB_STEP_CASE(M, F, B)                                                           \
+  if (moving == M && force == F && bf == B) {                                            \
+    launch_step_t<T, M, F, B>(L, A, src, dst, i0, np, sring, dring, G);                 \
+    return;                                                                              \
+  }
   SLB_STEP_CASE(false, false, 0) SLB_STEP_CASE(false, false, 1) SLB_STEP_CASE(false, false, 2)
   SLB_STEP_CASE(false, true, 0) SLB_STEP_CASE(false, true, 1) SLB_STEP_CASE(false, true, 2)
   SLB_STEP_CASE(true, false, 0) SLB_STEP_CASE(true, false, 1) SLB_STEP_CASE(true, false, 2)
@@ -1354,13 +1390,109 @@ static void launch_step(slbd_lattice *L, const PhysArgs &A, bool moving, bool fo
 #undef SLB_STEP_CASE
 }
 
+/* Default bytes of one chunk's planes in the two-step sweep; 0: the sweep
+ * is off unless SLB_SWEEP names a chunk size. */
+#ifndef SLB_SWEEP_BYTES
+#define SLB_SWEEP_BYTES 0L
+#endif
+
+/* Planes per chunk of the two-step sweep (0: off), from SLB_SWEEP (planes)
+ * or the default byte budget of a chunk's intermediate planes. */
+static int sweep_planes(const slbd_lattice *L) {
+  const long pbytes = 15L * L->G.plane * (long)tsize(L->prec);
+  int P;
+  if (getenv("SLB_SWEEP")) {
+    P = atoi(getenv("SLB_SWEEP"));
+  } else {
+    const long budget = SLB_SWEEP_BYTES;
+    P = (int)(budget / pbytes);
+  }
+  if (P < 4) return 0;
+  /* the sliding window of P + 4 planes must fit the second buffer, and the
+   * ring must start each plane on a whole population tile */
+  if (L->esi < 2 * P + 2 || L->G.plane % POP_TILE != 0) return 0;
+  /* at most 256 launches per sweep */
+  if ((L->esi + P - 1) / P * 2 + 1 > 256) return 0;
+  return P;
+}
+
+/* Two steps as one sweep over the planes (temporal blocking through the
+ * device's last-level cache).  Step t+1 of plane i needs step t of planes
+ * i-1..i+1, which needs step t-1 of planes i-2..i+2.  The planes are taken
+ * in chunks of P: launch one computes step t of the chunk's planes from the
+ * live buffer A into a sliding window of P + 2 planes (+ planes 0 and 1) in
+ * the second buffer, launch two computes step t+1 of the planes one behind
+ * from that window back into A, in place: the step t-1 values it overwrites
+ * have no reader left, since the planes that read them (i-1..i+1 of step t)
+ * are done.  Planes esi-1 and 0 of step t+1 close the sweep, when step t of
+ * planes esi-2..1 exists.  The window of the latest chunk (a few tens of MB)
+ * is written and read back while it is still in the 256 MB last-level
+ * cache, so HBM sees the live buffer read once and written once per two
+ * steps.  Every node computes exactly what the one-step kernel computes:
+ * the results are bit-identical. */
+extern "C" int slbd_sweep_plan(int esi, int P, int *rows, int maxrows) {
+  if (P < 4 || esi < 2 * P + 2) return 0;
+  const int R = P + 2;
+  int n = 0;
+  auto put = [&](int phase, int i0, int np, int sring, int dring) {
+    if (rows && n < maxrows) {
+      int *r = rows + 5 * n;
+      r[0] = phase; r[1] = i0; r[2] = np; r[3] = sring; r[4] = dring;
+    }
+    n++;
+  };
+  for (int c0 = 0; c0 < esi; c0 += P) {
+    const int np1 = c0 + P <= esi ? P : esi - c0;
+    put(1, c0, np1, 0, R);
+    const int lo = c0 == 0 ? 1 : c0 - 1;
+    const int hi = c0 + np1 - 1 < esi - 1 ? c0 + np1 - 1 : esi - 1;
+    if (hi > lo) put(2, lo, hi - lo, R, 0);
+  }
+  put(2, esi - 1, 2, R, 0);
+  return n;
+}
+
+template <typename T>
+static void sweep2(slbd_lattice *L, const PhysArgs &A, bool moving, int bf, int P) {
+  void *Abuf = L->pops[L->cur], *Bbuf = L->pops[L->cur ^ 1];
+  Geom g1 = L->G, g2 = L->G;
+  g1.xcd_map = 0; g2.xcd_map = 0;
+  g1.nt_store = L->sweep_nt_mid;   /* window stores: cacheable by default */
+  g2.nt_load = 0;
+  int rows[5 * 256];
+  const int n = slbd_sweep_plan(L->esi, P, rows, 256);
+  if (n < 1 || n > 256) {
+    fprintf(stderr, "sunlightlb: two-step sweep plan of %d launches\n", n);
+    exit(EXIT_FAILURE);
+  }
+  for (int q = 0; q < n; q++) {
+    const int *r = rows + 5 * q;
+    if (r[0] == 1)
+      launch_step<T>(L, A, moving, false, bf, Abuf, Bbuf, r[1], r[2], r[3], r[4], g1);
+    else
+      launch_step<T>(L, A, moving, false, bf, Bbuf, Abuf, r[1], r[2], r[3], r[4], g2);
+  }
+}
+
 extern "C" void slbd_step(slbd_lattice *L, const slbd_params *P, int nsteps, int force) {
   const PhysArgs A = phys(P);
   const int bf = bf_mode(L, P);
   if (force) ensure_f(L);
-  for (int s = 0; s < nsteps; s++) {
-    if (L->prec == 32) launch_step<float>(L, A, P->moving != 0, force != 0, bf);
-    else launch_step<double>(L, A, P->moving != 0, force != 0, bf);
+  const int sw = force ? 0 : sweep_planes(L);
+  int s = 0;
+  if (sw > 0) {
+    for (; s + 2 <= nsteps; s += 2) {
+      if (L->prec == 32) sweep2<float>(L, A, P->moving != 0, bf, sw);
+      else sweep2<double>(L, A, P->moving != 0, bf, sw);
+    }
+  }
+  for (; s < nsteps; s++) {
+    void *src = L->pops[L->cur], *dst = L->pops[L->cur ^ 1];
+    if (L->prec == 32)
+      launch_step<float>(L, A, P->moving != 0, force != 0, bf, src, dst, 0, L->esi, 0, 0, L->G);
+    else
+      launch_step<double>(L, A, P->moving != 0, force != 0, bf, src, dst, 0, L->esi, 0, 0, L->G);
+    L->cur ^= 1;
   }
 }
 
@@ -2111,10 +2243,16 @@ __global__ void k_inject(T *__restrict__ a, const uint16_t *__restrict__ flags,
 extern "C" long slbd_nodes(const slbd_lattice *L) { return L->N; }
 extern "C" void slbd_dims(const slbd_lattice *L, int *d) { d[0] = L->esi; d[1] = L->esj; d[2] = L->esk; }
 
-/* features of the lattice's solid map into a C-channel BF16 device tensor */
-extern "C" double slbd_features(slbd_lattice *L, uint16_t *X, int C) {
-  int *buf[3];
-  for (int b = 0; b < 3; b++) HIPCHECK(hipMalloc(&buf[b], sizeof(int) * L->N));
+__global__ void k_zero_f64(double *__restrict__ p, int n) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < n) p[q] = 0.0;
+}
+
+/* Features of the lattice's solid map into a C-channel BF16 device tensor,
+ * with the jump-flooding seed buffers buf[0..2] (N ints each) and the two
+ * accumulators acc[0..1] supplied by the caller.  Returns ell. */
+static double features_run(slbd_lattice *L, uint16_t *X, int C, int *const buf[3],
+                           double *acc) {
   int maxd = L->esi > L->esj ? L->esi : L->esj;
   maxd = maxd > L->esk ? maxd : L->esk;
   int s0 = 1;
@@ -2137,21 +2275,30 @@ extern "C" double slbd_features(slbd_lattice *L, uint16_t *X, int C) {
     res[phase] = pong;   /* ping is free for the next phase */
   }
   /* normalising length: mean distance from a fluid node to the solid */
-  double *acc, h[2];
-  HIPCHECK(hipMalloc(&acc, 2 * sizeof(double)));
-  HIPCHECK(hipMemsetAsync(acc, 0, 2 * sizeof(double), L->stream));
+  double h[2];
+  hipLaunchKernelGGL(k_zero_f64, dim3(1), dim3(64), 0, L->stream, acc, 2);
+  LAUNCHCHECK();
   hipLaunchKernelGGL(k_fluid_dist, dim3((unsigned)red_blocks(L->N)), dim3(256), 0, L->stream,
                      L->flags, res[0], acc, L->G);
   LAUNCHCHECK();
   HIPCHECK(hipMemcpyAsync(h, acc, 2 * sizeof(double), hipMemcpyDeviceToHost, L->stream));
   HIPCHECK(hipStreamSynchronize(L->stream));
-  HIPCHECK(hipFree(acc));
   const double ell = (h[1] > 0.0 && h[0] > 0.0) ? h[0] / h[1] : 1.0;
   hipLaunchKernelGGL(k_features, lin_grid(L->N), dim3(256), 0, L->stream, L->flags, res[0], res[1],
                      X, C, (float)(1.0 / ell), L->G);
   LAUNCHCHECK();
   HIPCHECK(hipStreamSynchronize(L->stream));
+  return ell;
+}
+
+extern "C" double slbd_features(slbd_lattice *L, uint16_t *X, int C) {
+  int *buf[3];
+  double *acc;
+  for (int b = 0; b < 3; b++) HIPCHECK(hipMalloc(&buf[b], sizeof(int) * L->N));
+  HIPCHECK(hipMalloc(&acc, 2 * sizeof(double)));
+  const double ell = features_run(L, X, C, buf, acc);
   for (int b = 0; b < 3; b++) HIPCHECK(hipFree(buf[b]));
+  HIPCHECK(hipFree(acc));
   return ell;
 }
 

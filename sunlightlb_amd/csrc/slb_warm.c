@@ -829,6 +829,11 @@ slb_warmnet *slb_warm_load(const char *path) {
            fread(&np, sizeof(long), 1, f) == 1 && np == tmp.nparam;
   for (int L = 0; ok && L < NLAYER; L++)
     if (hdr[1 + 2 * L] != LCIN[L] || hdr[2 + 2 * L] != LCOUT[L]) ok = 0;
+  /* curve lengths index the fixed-size curve tables; intervals are step
+   * counts */
+  if (ok && (ncurve < 0 || ncurve > WN_CURVE || nwcurve < 0 || nwcurve > WN_CURVE ||
+             cint < 0 || wcint < 0 || hdr[2 * NLAYER + 1] < 0))
+    ok = 0;
   float *hw = ok ? malloc(sizeof(float) * np) : NULL;
   if (ok && (!hw || fread(hw, sizeof(float), np, f) != (size_t)np)) ok = 0;
   fclose(f);

@@ -7,6 +7,7 @@ Run here, where the reference sources exist:
 
     make -C oracle ref && python tests/golden/make_golden.py
     python tests/golden/make_golden.py packs      # only the workload-size tables
+    python tests/golden/make_golden.py pack256    # only the 256^3 permeability pack
 
 Every lattice is non-cubic so that a transposed index cannot pass.  Body
 forces use the reference's current API (zerobf + addbf on fluid nodes).
@@ -586,10 +587,137 @@ def case_packs(R):
     print("wrote packs")
 
 
+MASK64 = (1 << 64) - 1
+
+
+def splitmix64(state):
+    """One draw of splitmix64 (Steele, Lea, Flood 2014): returns (new state,
+    output)."""
+    state = (state + 0x9E3779B97F4A7C15) & MASK64
+    z = state
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & MASK64
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & MASK64
+    return state, z ^ (z >> 31)
+
+
+def sphere_nodes(dims, R, o):
+    """Lattice nodes inside the box with (i-ox)^2 + (j-oy)^2 + (k-oz)^2 < R^2
+    (FP64, squares summed left to right: the insertsph predicate,
+    src/lbobjs.c:212-218)."""
+    ax = []
+    for d in range(3):
+        lo = max(0, int(np.floor(o[d] - R)) - 1)
+        hi = min(dims[d] - 1, int(np.ceil(o[d] + R)) + 1)
+        ax.append(np.arange(lo, hi + 1, dtype=np.float64) - o[d])
+    dx, dy, dz = ax
+    r2 = (dx[:, None, None] * dx[:, None, None] + dy[None, :, None] * dy[None, :, None]) \
+        + dz[None, None, :] * dz[None, None, :]
+    return int((r2 < R * R).sum())
+
+
+def rsa_pack_restated(dims, R, phi, seed, maxtrials=50_000_000):
+    """Random sequential addition of radius-R spheres in a periodic box, as
+    the library's slb_rsa_pack defines it: centres (u01 x, u01 y, u01 z)
+    from one splitmix64 stream, u01 = (z >> 11) 2^-53; a trial is rejected
+    when the minimum-image distance (dx -= L floor(dx/L + 1/2)) to a placed
+    centre is below 2R (squared, summed x, y, z); each accepted sphere is
+    inserted at every periodic image whose bounding box meets the box
+    (a, b, e in -1..1, x slowest); stop once the inserted nodes reach
+    ceil(phi N).  Returns the centres and the image rows (R, x, y, z) in
+    insertion order.  A restatement for the fixture, independent of the
+    library's C code, so that the library's generator is checked too."""
+    L = [float(d) for d in dims]
+    nc = [int(np.floor(L[d] / (2.0 * R))) for d in range(3)]
+    nc = [1 if c < 3 else c for c in nc]
+    cs = [L[d] / nc[d] for d in range(3)]
+    cells = {}
+    N = dims[0] * dims[1] * dims[2]
+    target = int(np.ceil(phi * N))
+    st, solid, centres, images = seed & MASK64, 0, [], []
+    d2min = 4.0 * R * R
+
+    def cell(c):
+        return tuple(min(max(int(np.floor(c[d] / cs[d])), 0), nc[d] - 1) for d in range(3))
+
+    t = 0
+    while t < maxtrials and solid < target:
+        t += 1
+        c = []
+        for d in range(3):
+            st, z = splitmix64(st)
+            c.append((z >> 11) * (1.0 / 9007199254740992.0) * L[d])
+        ci = cell(c)
+        hit = False
+        for a in (-1, 0, 1):
+            for b in (-1, 0, 1):
+                for e in (-1, 0, 1):
+                    key = ((ci[0] + a) % nc[0], (ci[1] + b) % nc[1], (ci[2] + e) % nc[2])
+                    for q in cells.get(key, ()):
+                        r2 = 0.0
+                        for d in range(3):
+                            dx = c[d] - centres[q][d]
+                            dx -= L[d] * np.floor(dx / L[d] + 0.5)
+                            r2 += dx * dx
+                        if r2 < d2min:
+                            hit = True
+                            break
+                    if hit:
+                        break
+                if hit:
+                    break
+            if hit:
+                break
+        if hit:
+            continue
+        cells.setdefault(ci, []).append(len(centres))
+        centres.append(c)
+        for a in (-1, 0, 1):
+            for b in (-1, 0, 1):
+                for e in (-1, 0, 1):
+                    o = (c[0] + a * L[0], c[1] + b * L[1], c[2] + e * L[2])
+                    if any(o[d] + R < 0.0 or o[d] - R > L[d] for d in range(3)):
+                        continue
+                    ns = sphere_nodes(dims, R, o)
+                    if ns == 0:
+                        continue
+                    images.append((R, *o))
+                    solid += ns
+    return centres, images, t
+
+
+def case_pack256(R):
+    """The 256^3 permeability pack (radius 8, lattice solid fraction 0.35,
+    seed 20261202): centres from the restated generator, solid map and
+    boundary-link mask written by the reference library (insertsph of every
+    image in order, then flattenobj), stored as SHA-256 digests."""
+    dims, Rad, phi, seed = (256, 256, 256), 8.0, 0.35, 20261202
+    centres, images, trials = rsa_pack_restated(dims, Rad, phi, seed)
+    print(f"pack256: {len(centres)} spheres, {len(images)} images, {trials} trials")
+    h = R.new(*dims)
+    placed = sum(1 if R.L.insertsph(h, *row) else 0 for row in images)
+    R.L.syncobjs(h)
+    R.L.flattenobj(h)
+    R.L.syncobjs(h)
+    st = state(R, h)
+    rec = dict(dims=list(dims), radius=Rad, phi=phi, seed=seed, trials=trials,
+               spheres=len(centres), images=len(images), placed=placed,
+               solid=int((st["objid"] >= 0).sum()),
+               flat_mask_sha256=sha(st["mask"]), flat_objid_sha256=sha(st["objid"]),
+               countfluid=int(R.L.countfluid(h)),
+               countmask_links=int(R.L.countmask(h, b"links")),
+               centres=[[float(x) for x in c] for c in centres])
+    with gzip.open(os.path.join(OUT, "pack256.json.gz"), "wt") as f:
+        json.dump(rec, f)
+    print("wrote pack256", {k: v for k, v in rec.items() if k != "centres"})
+
+
 def main():
     R = RefLib(64)
     if sys.argv[1:] == ["packs"]:
         case_packs(R)
+        return
+    if sys.argv[1:] == ["pack256"]:
+        case_pack256(R)
         return
     R32 = RefLib(32)
     case_tables(R)

@@ -141,3 +141,105 @@ def test_batch_autotune_cache(slb, tmp_path):
     finally:
         slb.slb_batch_free(b2)
     assert len(open(path.decode()).read().strip().splitlines()) == 1
+
+
+def test_batch_graph_replay_after_odd_run(slb):
+    """A captured step loop replays the buffers it was captured with: after
+    an odd number of plain steps the other buffer is live, and a replay
+    must use the loop captured from that buffer.  33 + 32 steps equal 65
+    steps in one call and the single-lattice run."""
+    b1, vels = make_batch(slb)
+    b2, _ = make_batch(slb)
+    try:
+        slb.slb_batch_run(b1, 33)      # one graph replay + one plain step
+        slb.slb_batch_run(b1, 32)      # replay from the other buffer
+        slb.slb_batch_measure(b1)
+        slb.slb_batch_run(b2, 65)
+        slb.slb_batch_measure(b2)
+        for q, box in enumerate(BOXES):
+            N = int(np.prod(box["dims"]))
+            for r in range(NRHS):
+                a1, a2 = np.zeros(15 * N), np.zeros(15 * N)
+                slb.slb_batch_get_pops(b1, q, r, a1.ctypes.data_as(C.POINTER(C.c_double)))
+                slb.slb_batch_get_pops(b2, q, r, a2.ctypes.data_as(C.POINTER(C.c_double)))
+                assert np.array_equal(a1, a2), (q, r)
+                a_ref, f_ref, t_ref = run_alone(box, vels[q][r], 65)
+                assert rel_l2(a1, a_ref) < 1e-12, (q, r)
+                nobj = len(f_ref)
+                frc, trq = np.zeros(3 * nobj), np.zeros(3 * nobj)
+                slb.slb_batch_forces(b1, q, r, frc.ctypes.data_as(C.POINTER(C.c_double)),
+                                     trq.ctypes.data_as(C.POINTER(C.c_double)))
+                assert rel_l2(frc, f_ref) < 1e-12 and rel_l2(trq, t_ref) < 1e-12, (q, r)
+    finally:
+        slb.slb_batch_free(b1)
+        slb.slb_batch_free(b2)
+
+
+def test_batch_autotune_after_odd_run_then_replay(slb):
+    """The autotuner times candidate shapes and returns the batch to rest
+    with buffer 0 live; after an odd run before it, the replayed step loop
+    must still start from the live buffer."""
+    b, vels = make_batch(slb)
+    try:
+        slb.slb_batch_run(b, 33)
+        assert slb.slb_batch_autotune(b, None) == 0
+        slb.slb_batch_run(b, 40)
+        slb.slb_batch_measure(b)
+        box = BOXES[1]
+        N = int(np.prod(box["dims"]))
+        a = np.zeros(15 * N)
+        slb.slb_batch_get_pops(b, 1, 2, a.ctypes.data_as(C.POINTER(C.c_double)))
+        a_ref, _, _ = run_alone(box, vels[1][2], 40)
+        assert rel_l2(a, a_ref) < 1e-12
+    finally:
+        slb.slb_batch_free(b)
+
+
+@pytest.mark.parametrize("cfg", [(256, 1, 0), (128, 4, 8)])
+def test_batch_against_reference_fixture(slb, cfg):
+    """The batched kernel against the reference library directly (no
+    single-lattice run in between): the moving sphere + moving wall case of
+    the golden fixture (src/lbkernel.c:330-454 moving-wall bounce-back,
+    :464-618 its force), populations after 40 steps and object force and
+    torque after measure(), FP64 to 1e-12.  A second right-hand side with
+    every velocity doubled must give exactly doubled results (Stokes flow
+    from rest is linear in the wall velocities)."""
+    from conftest import load_golden
+    g = load_golden("moving.npz")
+    dims = tuple(g["meta"]["dims"])
+    import sunlightlb_amd as s
+    lat = s.Lattice(*dims, precision=64)
+    L, h = lat.L, lat.h
+    lat.s.tau, lat.s.stokes_flow, lat.s.skip_solid, lat.s.objects_moving = 0.9, 1, 0, 1
+    wall = L.insertwall(h, b"z", 0)
+    sph = L.insertsph(h, 3.1, 6.3, 5.4, 6.2)
+    L.syncobjs(h)
+    objs = lat.objects()                     # list order: sphere (head), wall
+    assert len(objs) == 2 and objs[0].posx == 6.3
+    v1 = np.zeros((2, 6))
+    v1[0] = [0.012, -0.004, 0.0, 0.0, 0.003, 0.002]
+    v1[1] = [-0.005, 0.0, 0.0, 0.0, 0.0, 0.0]
+    vel = np.stack([v1, 2.0 * v1])
+    b = L.slb_batch_new(64)
+    try:
+        flat = np.ascontiguousarray(vel.reshape(-1))
+        assert L.slb_batch_add(b, h, 2, flat.ctypes.data_as(C.POINTER(C.c_double))) == 0
+        lat.free()
+        assert L.slb_batch_commit(b) == 0
+        L.slb_batch_set_config(b, *cfg)
+        L.slb_batch_run(b, 40)
+        N = int(np.prod(dims))
+        for r, scale in ((0, 1.0), (1, 2.0)):
+            a = np.zeros(15 * N)
+            L.slb_batch_get_pops(b, 0, r, a.ctypes.data_as(C.POINTER(C.c_double)))
+            assert rel_l2(a, scale * g["a_40"]) < 1e-12, r
+        L.slb_batch_measure(b)
+        for r, scale in ((0, 1.0), (1, 2.0)):
+            frc, trq = np.zeros(6), np.zeros(6)
+            assert L.slb_batch_forces(b, 0, r, frc.ctypes.data_as(C.POINTER(C.c_double)),
+                                      trq.ctypes.data_as(C.POINTER(C.c_double))) == 2
+            ref = g["meas_objs"]
+            assert rel_l2(frc, scale * ref[:, 9:12]) < 1e-12, r
+            assert rel_l2(trq, scale * ref[:, 12:15]) < 1e-12, r
+    finally:
+        L.slb_batch_free(b)

@@ -1,0 +1,106 @@
+"""The two-step sweep (two time steps per pass over the planes, the
+intermediate step held in a sliding plane window) against the one-step
+kernel: populations bit-identical after odd and even step counts, with
+moving walls, skip_solid, Navier-Stokes and per-node body forces, in FP64
+and FP32; and FP64 against the CPU oracle.  The sweep is chosen per call
+from SLB_SWEEP (planes per chunk; 0 = one launch per step)."""
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def build(dims, prec, moving=True, nonuniform=True):
+    import sunlightlb_amd as s
+    lat = s.Lattice(*dims, precision=prec)
+    L, h = lat.L, lat.h
+    lat.s.tau, lat.s.stokes_flow, lat.s.skip_solid = 0.7, 0, 1
+    lat.s.objects_moving = 1 if moving else 0
+    w = L.insertwall(h, b"y", 2)
+    s1 = L.insertsph(h, 5.3, 17.2, 9.5, 12.7)
+    s2 = L.insertsph(h, 4.1, 3.0, 20.1, 30.2)        # cut by the i = 0 face
+    if moving:
+        s1.contents.velz, s1.contents.angx = 0.02, 0.003
+        s2.contents.velx = -0.01
+        w.contents.velx = 0.005
+        L.vspread(h)
+    L.zerobf(h)
+    if nonuniform:
+        rng = np.random.default_rng(11)
+        lat.sync()
+        solid = lat.solid()
+        for nm, sc in (("bx", 2e-4), ("by", 5e-5), ("bz", 1e-4)):
+            v = s.view(getattr(lat.s, nm), lat.N)
+            v[:] = np.where(solid, 0.0, sc * (1 + rng.random(lat.N)))
+        L.slb_sync_device(h)
+    else:
+        L.addbf(h, 1e-4, 0.0, 2e-5)
+    return lat
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+@pytest.mark.parametrize("planes", [4, 5, 8])
+def test_sweep_bit_identical_to_single_steps(prec, planes, monkeypatch):
+    dims = (37, 32, 32)          # 1024-node planes; 37 = 4 chunks of 8 + 5
+    print(f"sweep: prec {prec} planes {planes} dims {dims}", flush=True)
+    monkeypatch.setenv("SLB_SWEEP", "0")
+    ref = build(dims, prec)
+    ref.step(1)
+    ref.L.slb_run(ref.h, 41)
+    a_ref = ref.pops()
+    monkeypatch.setenv("SLB_SWEEP", str(planes))
+    sw = build(dims, prec)
+    sw.step(1)
+    sw.L.slb_run(sw.h, 41)
+    a_sw = sw.pops()
+    assert sw.s.step == ref.s.step == 42
+    assert np.array_equal(a_sw, a_ref), (
+        f"max diff {np.abs(a_sw - a_ref).max():.3e}")
+    # measure() after the sweep: the force-collecting steps run one at a
+    # time; per-object sums use FP64 atomics (summation order varies at the
+    # last bit), node forces are bit-identical
+    ref.L.measure(ref.h)
+    sw.L.measure(sw.h)
+    for nm in ("fx", "fy", "fz"):
+        assert np.array_equal(ref.field(nm), sw.field(nm))
+    for o1, o2 in zip(ref.objects(), sw.objects()):
+        f1 = np.array([o1.frcx, o1.frcy, o1.frcz, o1.trqx, o1.trqy, o1.trqz])
+        f2 = np.array([o2.frcx, o2.frcy, o2.frcz, o2.trqx, o2.trqy, o2.trqz])
+        assert np.abs(f1 - f2).max() <= 1e-13 * np.abs(f1).max()
+    ref.free()
+    sw.free()
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+def test_sweep_uniform_force_static(prec, monkeypatch):
+    dims = (24, 32, 64)
+    monkeypatch.setenv("SLB_SWEEP", "0")
+    ref = build(dims, prec, moving=False, nonuniform=False)
+    ref.L.slb_run(ref.h, 30)
+    monkeypatch.setenv("SLB_SWEEP", "6")
+    sw = build(dims, prec, moving=False, nonuniform=False)
+    sw.L.slb_run(sw.h, 30)
+    assert np.array_equal(sw.pops(), ref.pops())
+    ref.free()
+    sw.free()
+
+
+def test_sweep_fp64_matches_oracle(oracle, monkeypatch):
+    """The sweep in FP64 against the CPU oracle (Navier-Stokes, skip_solid,
+    moving walls, per-node body force)."""
+    monkeypatch.setenv("SLB_SWEEP", "4")
+    dims = (20, 32, 32)
+    lat = build(dims, 64)
+    lat.L.slb_run(lat.h, 24)
+    from test_gpu_parity import dense_objid, objrows
+    objid = dense_objid(lat)
+    mask = oracle.makemask(objid, dims)
+    rows = objrows(lat)
+    u = oracle.vspread(objid, dims, rows[:, 0:3], rows[:, 3:6], rows[:, 6:9])
+    bb = [lat.field("bx"), lat.field("by"), lat.field("bz")]
+    ao = np.zeros(lat.N * 15)
+    oracle.lbstep(ao, mask, objid, dims, bb, 0.7, 0, 1, u=u, nsteps=24)
+    assert rel_l2(lat.pops(), ao) < 1e-12
+    lat.free()

@@ -24,6 +24,48 @@ def test_warm_weights_file_errors(tmp_path):
     assert not L.slb_warm_load(str(trunc).encode())
 
 
+def _weights_file(path, ncurve, nwcurve):
+    """A complete weights file in the layout of slb_warm_save (magic, layer
+    table, fallback threshold, both curves, zero parameters) with the given
+    curve counts."""
+    import ctypes as C
+    import struct
+    L = lib()
+    nl = C.c_int(0)
+    cin = (C.c_int * 32)()
+    cout = (C.c_int * 32)()
+    bofs = (C.c_long * 32)()
+    L.slb_warm_layout(C.byref(nl), cin, cout, None, bofs)
+    nparam = bofs[nl.value - 1] + 64
+    hdr = [nl.value] + [v for q in range(nl.value) for v in (cin[q], cout[q])] + [0]
+    blob = b"SLBWNET4" + struct.pack(f"<{len(hdr)}i", *hdr) + struct.pack("<d", 0.0)
+    blob += struct.pack("<ii", ncurve, 100) + bytes(8 * 256)
+    blob += struct.pack("<ii", nwcurve, 100) + bytes(8 * 256)
+    blob += struct.pack("<q", nparam) + bytes(4 * nparam)
+    path.write_bytes(blob)
+
+
+def test_warm_weights_file_with_bad_curve_counts(tmp_path):
+    """A curve count beyond the fixed curve table is refused before any
+    parameter is read or any device memory is touched (a subprocess, so
+    that a regression which went on to the device cannot end this one)."""
+    def load(nc, nw):
+        f = tmp_path / f"curve_{nc}_{nw}.bin"
+        _weights_file(f, nc, nw)
+        code = ("import sunlightlb_amd as s; L = s.lib(); "
+                f"print('loaded' if L.slb_warm_load({str(f)!r}.encode()) else 'refused')")
+        return subprocess.run([sys.executable, "-c", code], capture_output=True, text=True,
+                              timeout=120, cwd=ROOT)
+
+    for nc, nw in ((100000, 10), (10, 100000), (-1, 10), (10, -5)):
+        p = load(nc, nw)
+        assert p.returncode == 0 and p.stdout.strip() == "refused", (nc, nw, p.stdout, p.stderr)
+    # the same file with counts inside the table is not refused: it goes on
+    # to the device (which a host without a GPU reports as an error)
+    p = load(256, 12)
+    assert "refused" not in p.stdout, p.stdout
+
+
 def test_bench_refuses_several_gpus():
     p = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
                        capture_output=True, text=True, timeout=120)
