@@ -229,6 +229,17 @@ void   slb_warm_sample_stats(const slb_warmnet *net, int i, double out[4]);
  * moments writes the same targets in FP32; inject builds populations from
  * scaled moments m (device FP32 [n][channels >= 10]) in the given mode. */
 double slb_warm_features(LBSys lb, void *x_dev, int channels);
+/* slb_warm_features without any device allocation: no counterpart in the
+ * reference; for callers that cannot allocate device memory (code under
+ * HIP graph capture around it, tight serving loops, harnesses that forbid
+ * hipMalloc).  The jump-flooding seed buffers live in a static device
+ * scratch of the library (24 MiB + 256 B per GPU, held until the process
+ * ends), so lattices are limited to 2^21 nodes (128^3): a larger lattice is
+ * refused with a message on stderr and -1 is returned, x_dev untouched.
+ * Same result as slb_warm_features, bit for bit.  Every call on one GPU
+ * shares the scratch: calls that overlap on one GPU (several host threads)
+ * race and must be serialised by the caller. */
+double slb_warm_features_noalloc(LBSys lb, void *x_dev, int channels);
 void   slb_warm_targets(LBSys lb, void *t_dev, int channels, double ell);
 void   slb_warm_moments(LBSys lb, float *m_dev, int channels, double ell);
 void   slb_warm_inject(LBSys lb, const float *m_dev, int channels, int mode, double ell);

@@ -142,6 +142,14 @@ void slbd_dims(const slbd_lattice *L, int *d);
  * flooding) with ell the mean distance from a fluid node to the nearest
  * solid node (returned); C-channel BF16 device tensor */
 double slbd_features(slbd_lattice *L, uint16_t *X, int C);
+/* The same features with the seed buffers in the library's static device
+ * scratch (no device allocation): lattices of at most
+ * SLBD_FEATURES_MAX_NODES nodes; returns -1 (and prints why) beyond that.
+ * slbd_scratch_init resolves the scratch's device address on the current
+ * device (called at lattice creation). */
+#define SLBD_FEATURES_MAX_NODES (1L << 21)
+double slbd_features_noalloc(slbd_lattice *L, uint16_t *X, int C);
+void slbd_scratch_init(void);
 /* the 10 moments of the live state on fluid nodes (0 on solid), scaled per
  * group: 0..2 (j - b/2)/rho * sc[0], 3 drho * sc[1], 4..9 stress
  * (xx,yy,zz,xy,xz,yz) * sc[2]; BF16 [n][C] */

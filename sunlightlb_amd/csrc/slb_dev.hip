@@ -848,6 +848,7 @@ extern "C" slbd_lattice *slbd_create(int esi, int esj, int esk, int prec) {
     exit(EXIT_FAILURE);
   }
   if (prec != 32 && prec != 64) prec = 64;
+  slbd_scratch_init();
   slbd_lattice *L = (slbd_lattice *)calloc(1, sizeof(slbd_lattice));
   L->esi = esi; L->esj = esj; L->esk = esk; L->prec = prec;
   L->N = (long)esi * esj * esk;
@@ -2300,6 +2301,55 @@ extern "C" double slbd_features(slbd_lattice *L, uint16_t *X, int C) {
   for (int b = 0; b < 3; b++) HIPCHECK(hipFree(buf[b]));
   HIPCHECK(hipFree(acc));
   return ell;
+}
+
+/* Allocation-free features: the seed buffers and accumulators live in one
+ * file-scope device array, `scratch`, held by every GPU the library runs on
+ * until the process ends: 256 bytes of accumulators, then three seed
+ * buffers of SLBD_FEATURES_MAX_NODES ints (24 MiB + 256 B).  In host code
+ * `scratch` names a host-side stand-in, not device memory: its device
+ * address is resolved once per device with hipGetSymbolAddress (at lattice
+ * creation, or the first call on a device) and only that address is passed
+ * to kernels.  Every call on one GPU shares the array; calls that overlap
+ * on one GPU (two host threads, or two streams in flight) race. */
+#define SCRATCH_ACC_BYTES 256
+static __device__ __align__(256) unsigned char
+    scratch[SCRATCH_ACC_BYTES + 3 * sizeof(int) * (size_t)SLBD_FEATURES_MAX_NODES];
+#define SCRATCH_MAX_DEVICES 64
+static unsigned char *scratch_dev[SCRATCH_MAX_DEVICES];
+
+/* Device address of `scratch` on the current device (resolved on first use
+ * there); NULL when it cannot be resolved. */
+static unsigned char *scratch_ptr(void) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= SCRATCH_MAX_DEVICES) return NULL;
+  if (!scratch_dev[dev]) {
+    void *p = NULL;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) {
+      fprintf(stderr, "sunlightlb: hipGetSymbolAddress(scratch) failed on device %d: %s\n",
+              dev, hipGetErrorString(e));
+      return NULL;
+    }
+    scratch_dev[dev] = (unsigned char *)p;
+  }
+  return scratch_dev[dev];
+}
+
+extern "C" void slbd_scratch_init(void) { (void)scratch_ptr(); }
+
+extern "C" double slbd_features_noalloc(slbd_lattice *L, uint16_t *X, int C) {
+  if (L->N > (long)SLBD_FEATURES_MAX_NODES) {
+    fprintf(stderr, "slb_warm_features_noalloc: a lattice of %ld nodes exceeds the static "
+            "scratch of %ld nodes\n", L->N, (long)SLBD_FEATURES_MAX_NODES);
+    return -1.0;
+  }
+  unsigned char *base = scratch_ptr();
+  if (!base) return -1.0;
+  int *buf[3];
+  for (int b = 0; b < 3; b++)
+    buf[b] = (int *)(base + SCRATCH_ACC_BYTES + (size_t)b * sizeof(int) * SLBD_FEATURES_MAX_NODES);
+  return features_run(L, X, C, buf, (double *)base);
 }
 
 template <typename O>

@@ -481,6 +481,14 @@ double slb_warm_features(LBSys lb, void *x, int channels) {
   return slbd_features(PRIV(lb)->dev, (uint16_t *)x, channels);
 }
 
+double slb_warm_features_noalloc(LBSys lb, void *x, int channels) {
+  slbd_params P;
+  slb_params(lb, &P);
+  slb_dev_geom(lb);
+  slbn_sync();
+  return slbd_features_noalloc(PRIV(lb)->dev, (uint16_t *)x, channels);
+}
+
 void slb_warm_moments(LBSys lb, float *m, int channels, double ell) {
   slbd_params P;
   float sc[3];
