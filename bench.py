@@ -34,6 +34,15 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def host_cores():
+    """Cores this process may run on (the reference is serial and uses one;
+    the host's count is reported beside it)."""
+    try:
+        return len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        return os.cpu_count() or 1
+
+
 def build_pack(L, n, R, phi, seed, force):
     h = L.newlb(n, n, n, b"null")
     s = h.contents
@@ -76,10 +85,11 @@ def cpu_baseline(L, h, n, force, steps):
         R.L.lbstep(rh)
     dt = time.perf_counter() - t0
     mlups = n ** 3 * steps / dt / 1e6
-    return dict(value=round(mlups, 3), unit="MLUPS", cores=1, kind="reference",
-                sample=f"reference SunlightLB (-D_SINGLE, gcc -O3 -ffast-math, serial) on the "
-                       f"same {n}^3 pack, {steps} lbstep calls after one untimed step, "
-                       f"{dt:.1f} s")
+    return dict(value=round(mlups, 3), unit="MLUPS", cores=1, host_cores=host_cores(),
+                kind="reference",
+                sample=f"reference SunlightLB (-D_SINGLE, gcc -O3 -ffast-math, serial: 1 of "
+                       f"{host_cores()} host cores) on the same {n}^3 pack, {steps} lbstep "
+                       f"calls after one untimed step, {dt:.1f} s")
 
 
 def ref_from_map(R, L, h, n):
@@ -116,7 +126,7 @@ def cpu_mp_baseline(L, h, n, steps):
         R.L.mpstep(mp)
     dt = time.perf_counter() - t0
     return dict(value=round(n ** 3 * steps / dt / 1e6, 3), unit="MLUPS", cores=1,
-                kind="reference",
+                host_cores=host_cores(), kind="reference",
                 sample=f"reference mpstep (-D_SINGLE, serial) on the same {n}^3 map, {steps} "
                        f"steps after one untimed step, {dt:.1f} s")
 
@@ -155,7 +165,7 @@ def cpu_batch_baseline(boxes, steps, u0, tau, nbox=16):
         dt += time.perf_counter() - t0
         nodes += Lb ** 3
     return dict(value=round(nodes * steps / dt / 1e6, 3), unit="MLUPS", cores=1,
-                kind="reference",
+                host_cores=host_cores(), kind="reference",
                 sample=f"reference lbstep with moving walls (-D_SINGLE, serial) on the first "
                        f"{min(nbox, len(boxes))} boxes of the batch, one right-hand side each, "
                        f"{steps} steps per box, {dt:.1f} s")
@@ -528,18 +538,11 @@ def solve_loop(L, h, tol, interval, maxsteps):
     return steps, hist, r
 
 
-def warm_workload(L, args):
-    """Learned warm start: generate and solve training packs on the GPU,
-    train the network on them, then solve unseen packs cold and warm to the
-    same residual and compare steps, wall time and permeability."""
+def train_network(L, args, net, res, rng, n, F, workers):
+    """Training data (cold solves of seeded packs to the residual
+    tolerance, each noted as a cold residual curve and added as a sample)
+    and Adam training of the network."""
     import numpy as np
-    rng = np.random.default_rng(args.seed + 5)
-    n = args.warm_size
-    F = args.force
-    eta = 72.0 * (0.75 - 0.5) / 3.0
-    workers = max(1, min(16, os.cpu_count() or 1))
-    net = L.slb_warm_new(args.seed)
-    res = {"train_packs": args.warm_train, "train_size": n, "workers": workers}
     # ---- training data: cold solves of seeded packs ----
     t0 = time.perf_counter()
     cold_steps = []
@@ -571,24 +574,52 @@ def warm_workload(L, args):
                 "ms_per_iter": round(dt / args.warm_iters * 1e3, 3), "loss_curve": curve,
                 "final_loss": last, "fallback_probe_steps": pr.value,
                 "fallback_probe_residual": pres.value})
+
+
+def warm_workload(L, args):
+    """Learned warm start: generate and solve training packs on the GPU,
+    train the network on them, then solve unseen packs cold and warm to the
+    same residual and compare steps, wall time and permeability."""
+    import numpy as np
+    rng = np.random.default_rng(args.seed + 5)
+    n = args.warm_size
+    F = args.force
+    eta = 72.0 * (0.75 - 0.5) / 3.0
+    workers = max(1, min(16, os.cpu_count() or 1))
+    proj = tuple(int(x) for x in args.warm_proj.split(","))
+    if args.warm_load:
+        net = L.slb_warm_load(args.warm_load.encode())
+        if not net:
+            raise RuntimeError(f"cannot load warm-start weights {args.warm_load}")
+        res = {"weights": os.path.relpath(args.warm_load, ROOT), "workers": workers}
+    else:
+        net = L.slb_warm_new(args.seed)
+        res = {"train_packs": args.warm_train, "train_size": n, "workers": workers}
+        train_network(L, args, net, res, rng, n, F, workers)
     # ---- start-policy calibration: warm solves of unseen training-distribution
-    # packs, without fallback, for their median residual curve ----
-    t0 = time.perf_counter()
-    crng = np.random.default_rng(args.seed + 13)
-    calib = []
-    for solid, phi in warm_maps(L, n, warm_params(crng, args.warm_calib), workers):
-        h = warm_lattice(L, n, solid, F)
-        L.slb_warm_set_mode(net, 1)
-        L.slb_warm_start(net, h)
-        steps, hist, r = solve_loop(L, h, args.tol, args.interval, args.maxsteps)
-        L.slb_warm_note_warm(net, (C.c_double * len(hist))(*hist), len(hist), args.interval)
-        calib.append(steps)
-        L.slb_destroy(h)
-    res["calibration"] = {"packs": args.warm_calib, "seconds": round(time.perf_counter() - t0, 1),
-                          "warm_steps_median": float(np.median(calib)) if calib else None,
-                          "start_warm_at": {f"{tol:.0e}": L.slb_warm_policy(net, tol, args.interval)
-                                            for tol in (1e-3, 1e-4, 1e-5, 1e-6)}}
-    log(f"warm: start policy {res['calibration']['start_warm_at']}")
+    # packs, without fallback, for their median residual curve (with the
+    # projection when --warm-proj-calib; slb_warm_solve then projects too) ----
+    calib_proj = proj if args.warm_proj_calib else (0, 0)
+    L.slb_warm_set_projection(net, *calib_proj)
+    if args.warm_calib > 0:
+        t0 = time.perf_counter()
+        crng = np.random.default_rng(args.seed + 13)
+        calib = []
+        for solid, phi in warm_maps(L, n, warm_params(crng, args.warm_calib), workers):
+            h = warm_lattice(L, n, solid, F)
+            L.slb_warm_set_mode(net, 1)
+            L.slb_warm_start(net, h)
+            steps, hist, r = solve_loop(L, h, args.tol, args.interval, args.maxsteps)
+            L.slb_warm_note_warm(net, (C.c_double * len(hist))(*hist), len(hist), args.interval)
+            calib.append(steps)
+            L.slb_destroy(h)
+        res["calibration"] = {"packs": args.warm_calib,
+                              "seconds": round(time.perf_counter() - t0, 1),
+                              "projection": list(calib_proj),
+                              "warm_steps_median": float(np.median(calib)) if calib else None}
+    res["start_warm_at"] = {f"{tol:.0e}": L.slb_warm_policy(net, tol, args.interval)
+                            for tol in (1e-3, 1e-4, 1e-5, 1e-6)}
+    log(f"warm: start policy {res['start_warm_at']}")
     if args.warm_save:
         L.slb_warm_save(net, args.warm_save.encode())
     # ---- evaluation on unseen packs ----
@@ -601,6 +632,8 @@ def warm_workload(L, args):
             row = {"n": size, "R": round(p[0], 3), "phi": round(phi, 4), "overlap": p[2]}
             first = len([r for r in rows if r["n"] == size]) < 4
             modes = ["cold", "warm", "warm_trace", "equilibrium_trace"]
+            if proj[0] > 0:
+                modes += ["proj_trace"]
             if args.aa_window > 0:
                 modes += ["cold_aa", "warm_aa"]
             for mode in modes:
@@ -615,6 +648,7 @@ def warm_workload(L, args):
                     used = 0
                 elif mode == "warm":
                     L.slb_warm_set_mode(net, 1)
+                    L.slb_warm_set_projection(net, *calib_proj)
                     rr, uw = C.c_double(0.0), C.c_int(0)
                     steps = L.slb_warm_solve(net, h, args.tol, args.interval, args.maxsteps,
                                              C.byref(rr), C.byref(uw))
@@ -625,6 +659,7 @@ def warm_workload(L, args):
                     used = 0
                     if mode == "warm_aa":
                         L.slb_warm_set_mode(net, 1)
+                        L.slb_warm_set_projection(net, *proj)
                         L.slb_warm_start(net, h)
                         used = 1
                     rr = C.c_double(0.0)
@@ -632,8 +667,10 @@ def warm_workload(L, args):
                                                      args.aa_window, C.byref(rr))
                     r = rr.value
                 else:
-                    # the warm start without the fallback, for its residual history
-                    L.slb_warm_set_mode(net, 1 if mode == "warm_trace" else 0)
+                    # the warm start without the fallback, for its residual
+                    # history (proj_trace: with the divergence projection)
+                    L.slb_warm_set_mode(net, 0 if mode == "equilibrium_trace" else 1)
+                    L.slb_warm_set_projection(net, *(proj if mode == "proj_trace" else (0, 0)))
                     L.slb_warm_start(net, h)
                     steps, hist, r = solve_loop(L, h, args.tol, args.interval, args.maxsteps)
                     used = 1
@@ -652,7 +689,8 @@ def warm_workload(L, args):
                     row["pred_rel_mse"] = L.slb_warm_eval(net, h, grp)
                     row["pred_rel_mse_groups"] = list(grp)
                 L.slb_destroy(h)
-            for mode in ("warm", "warm_trace", "equilibrium_trace"):
+            for mode in ("warm", "warm_trace", "equilibrium_trace", "proj_trace", "cold_aa",
+                         "warm_aa"):
                 if mode in row:
                     row[mode]["k_rel_diff"] = (abs(row[mode]["permeability"] - row["cold"]["permeability"])
                                                / abs(row["cold"]["permeability"]))
@@ -699,6 +737,14 @@ def warm_workload(L, args):
                                         for t in ("1e-03", "1e-04", "1e-05", "1e-06")},
             "equilibrium_steps_ratio": {t: ratio(f"steps_to_{t}", "equilibrium_trace", eq)
                                         for t in ("1e-03", "1e-04", "1e-05", "1e-06")} if eq else None,
+            "projection": list(proj) if proj[0] > 0 else None,
+            "projection_steps_ratio": {t: ratio(f"steps_to_{t}", "proj_trace")
+                                       for t in ("1e-03", "1e-04", "1e-05", "1e-06")}
+            if proj[0] > 0 else None,
+            "projection_wall_ratio": (float(cs.sum() / sum(r["proj_trace"]["seconds"] for r in rs))
+                                      if proj[0] > 0 else None),
+            "projection_k_rel_diff_max": (float(max(r["proj_trace"]["k_rel_diff"] for r in rs))
+                                          if proj[0] > 0 else None),
             "pred_rel_mse_mean": float(np.mean([r["pred_rel_mse"] for r in rs])),
         }
         if args.aa_window > 0:
@@ -736,14 +782,15 @@ def main():
     ap.add_argument("--cpu-mp-steps", type=int, default=100)
     ap.add_argument("--cpu-batch-steps", type=int, default=100)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--converge", action="store_true",
-                    help="also run FP64 and FP32 to the velocity residual tolerance and "
-                         "compare permeabilities")
+    ap.add_argument("--no-converge", action="store_true",
+                    help="skip the cold-start solves of the pack to the velocity residual "
+                         "tolerance (FP64 and FP32, plain and Anderson-mixed) and the "
+                         "permeability comparison")
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--interval", type=int, default=100)
     ap.add_argument("--maxsteps", type=int, default=200000)
     ap.add_argument("--aa-window", type=int, default=5,
-                    help="Anderson window of the accelerated solves in --converge (0: none)")
+                    help="Anderson window of the accelerated cold-start solves (0: none)")
     ap.add_argument("--mp", action="store_true",
                     help="also run the passive-scalar workload: 128^3 pack (R 6, phi 0.25) "
                          "at Peclet 10, adsorption 0.01, 5000 moment-propagation steps")
@@ -766,6 +813,19 @@ def main():
                     help="unseen packs solved warm after training for the start policy")
     ap.add_argument("--warm-eval-sizes", default="128,256")
     ap.add_argument("--warm-save", default="")
+    ap.add_argument("--warm-load", default="",
+                    help="evaluate the weights in this file instead of training a network")
+    ap.add_argument("--warm-proj", default="200,3",
+                    help="divergence projection of the warm start in the evaluation: CG "
+                         "iterations per pass, passes (0,0: none)")
+    ap.add_argument("--warm-proj-calib", action="store_true",
+                    help="calibrate the start policy (and run slb_warm_solve) with the projection")
+    ap.add_argument("--warm-weights", default=os.path.join(ROOT, "sunlightlb_amd", "weights",
+                                                           "warm_unet128.bin"),
+                    help="weights for the default warm-vs-cold leg (skipped when absent)")
+    ap.add_argument("--warm-default-packs", type=int, default=6,
+                    help="unseen 128^3 packs of the default warm-vs-cold leg")
+    ap.add_argument("--no-warm", action="store_true", help="skip the default warm-vs-cold leg")
     ap.add_argument("--copy-test", action="store_true",
                     help="only run the plain device copy kernel (counter calibration)")
     args = ap.parse_args()
@@ -811,7 +871,7 @@ def main():
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(L, h, n, args.force, args.cpu_steps)
     conv = None
-    if args.converge:
+    if not args.no_converge:
         L.slb_destroy(h)
         conv = converge(L, args)
 
@@ -863,6 +923,13 @@ def main():
         out["batch"] = batch_workload(L, args)
     if args.warm:
         out["warm"] = warm_workload(L, args)
+    elif not args.no_warm and os.path.exists(args.warm_weights):
+        # the learned warm start on the committed weights: unseen 128^3 packs
+        # solved cold and warm to the residual tolerance
+        args.warm_load, args.warm_calib = args.warm_weights, 0
+        args.warm_eval, args.warm_eval_sizes = args.warm_default_packs, "128"
+        out["warm"] = warm_workload(L, args)
+        out["warm"].pop("eval_rows", None)
     print(json.dumps(out), file=json_out, flush=True)
 
 

@@ -256,6 +256,20 @@ double slb_warm_eval(slb_warmnet *net, LBSys lb, double *grp);
  * and body force. */
 void   slb_warm_predict(slb_warmnet *net, LBSys lb, float *m);
 void   slb_warm_set_mode(slb_warmnet *net, int mode);
+/* Divergence projection of the predicted velocity before injection
+ * (iters > 0 and passes > 0; iters = 0: off, the default).  The face
+ * fluxes (u(x) + u(x+e_a))/2 between fluid neighbours (none through walls,
+ * periodic wrap) lose their divergence: A phi = -div F by iters
+ * conjugate-gradient iterations, A the Laplacian of the graph of fluid
+ * nodes and open faces; then each fluid node's velocity component a loses
+ * the mean of phi's differences across its open faces along a.  That node
+ * correction removes the face divergence only approximately, so the whole
+ * step is repeated `passes` times.  slb_warm_project applies the same to
+ * the channels 0..2 of a device FP32 tensor m [n][channels] on lb's solid
+ * map and returns the last pass's final relative CG residual. */
+void   slb_warm_set_projection(slb_warmnet *net, int iters, int passes);
+void   slb_warm_get_projection(const slb_warmnet *net, int *iters, int *passes);
+double slb_warm_project(LBSys lb, float *m_dev, int channels, int iters, int passes);
 int    slb_warm_get_mode(const slb_warmnet *net);
 /* Initialise the populations from the prediction (the mode above). */
 void   slb_warm_start(slb_warmnet *net, LBSys lb);

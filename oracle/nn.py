@@ -315,3 +315,77 @@ def exact_distance(seed_mask):
     for s in seeds:
         best = np.minimum(best, _pdist2(shape, idx, np.full_like(idx, s)))
     return np.sqrt(best).reshape(shape)
+
+
+# --------------------------------------------------------------------------
+# Divergence projection of a predicted velocity field (slb_warm_project).
+
+def face_divergence(u, solid):
+    """div F(x) = sum_a [F(x, x+e_a) - F(x-e_a, x)] on fluid nodes (0 on
+    solid), face flux F(x, x+e_a) = (u_a(x) + u_a(x+e_a))/2 when both nodes
+    are fluid and 0 otherwise, periodic.  u[i, j, k, a], solid[i, j, k]."""
+    fl = ~solid
+    div = np.zeros(solid.shape)
+    for a in range(3):
+        nxt = np.roll(fl, -1, axis=a)
+        ua = u[..., a]
+        F = np.where(fl & nxt, 0.5 * (ua + np.roll(ua, -1, axis=a)), 0.0)   # F(x, x+e_a)
+        div += F - np.roll(F, 1, axis=a)
+    return np.where(fl, div, 0.0)
+
+
+def fluid_laplacian(solid):
+    """Sparse Laplacian of the graph of fluid nodes joined across open faces
+    (A phi(x) = sum_open (phi(x) - phi(nb)), zero rows on solid nodes)."""
+    import scipy.sparse as sp
+    shape = solid.shape
+    N = solid.size
+    idx = np.arange(N).reshape(shape)
+    fl = ~solid
+    rows, cols, vals = [], [], []
+    diag = np.zeros(N)
+    for a in range(3):
+        for sgn in (-1, 1):
+            nb = np.roll(idx, -sgn, axis=a)
+            open_ = (fl & np.roll(fl, -sgn, axis=a)).reshape(-1)
+            src = idx.reshape(-1)[open_]
+            dst = nb.reshape(-1)[open_]
+            rows.append(src)
+            cols.append(dst)
+            vals.append(-np.ones(len(src)))
+            np.add.at(diag, src, 1.0)
+    rows.append(np.arange(N))
+    cols.append(np.arange(N))
+    vals.append(diag)
+    return sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))),
+                         shape=(N, N))
+
+
+def project(u, solid, passes=1):
+    """The projection slb_warm_project defines, each pass solved to machine
+    precision (least squares on the singular Laplacian): phi with
+    A phi = -div F, then on each fluid node and axis a, u_a -= mean over the
+    open faces along a of phi's difference across the face (phi(x+e_a) -
+    phi(x) forward, phi(x) - phi(x-e_a) backward); repeated `passes` times.
+    Returns (u', phi of the last pass)."""
+    out, phi = u, None
+    for _ in range(passes):
+        out, phi = _project_pass(out, solid)
+    return out, phi
+
+
+def _project_pass(u, solid):
+    import scipy.sparse.linalg as sla
+    A = fluid_laplacian(solid)
+    rhs = -face_divergence(u, solid).reshape(-1)
+    phi = sla.lsqr(A, rhs, atol=1e-15, btol=1e-15, iter_lim=20000)[0].reshape(solid.shape)
+    fl = ~solid
+    out = u.copy()
+    for a in range(3):
+        fwd = np.roll(fl, -1, axis=a) & fl
+        bwd = np.roll(fl, 1, axis=a) & fl
+        d = np.where(fwd, np.roll(phi, -1, axis=a) - phi, 0.0) + \
+            np.where(bwd, phi - np.roll(phi, 1, axis=a), 0.0)
+        c = fwd.astype(int) + bwd.astype(int)
+        out[..., a] = np.where(fl & (c > 0), u[..., a] - d / np.maximum(c, 1), u[..., a])
+    return out, phi

@@ -186,7 +186,10 @@ SIGNATURES = {
     "slb_warm_sample_stats": (None, [C.c_void_p, I, RealP]),
     "slb_warm_set_mode": (None, [C.c_void_p, I]), "slb_warm_get_mode": (I, [C.c_void_p]),
     "slb_warm_features": (D, [LBSys, VP, I]),
-    "slb_warm_features_noalloc": (D, [LBSys, VP, I]), "slb_warm_targets": (None, [LBSys, VP, I, D]),
+    "slb_warm_features_noalloc": (D, [LBSys, VP, I]),
+    "slb_warm_set_projection": (None, [VP, I, I]),
+    "slb_warm_get_projection": (None, [VP, IntP, IntP]),
+    "slb_warm_project": (D, [LBSys, VP, I, I, I]), "slb_warm_targets": (None, [LBSys, VP, I, D]),
     "slb_warm_moments": (None, [LBSys, VP, I, D]), "slb_warm_inject": (None, [LBSys, VP, I, I, D]),
     "slb_warm_train": (D, [C.c_void_p, I, D, C.c_ulonglong, RealP]),
     "slb_warm_eval": (D, [C.c_void_p, LBSys, RealP]),

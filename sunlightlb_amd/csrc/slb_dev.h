@@ -148,6 +148,12 @@ double slbd_features(slbd_lattice *L, uint16_t *X, int C);
  * slbd_scratch_init resolves the scratch's device address on the current
  * device (called at lattice creation). */
 #define SLBD_FEATURES_MAX_NODES (1L << 21)
+/* Project the velocity channels 0..2 of the FP32 device tensor U [n][CU]
+ * towards the discretely mass-conserving fields of the fluid nodes: passes
+ * times (face divergence, iters conjugate-gradient iterations on the
+ * fluid-graph Laplacian, node correction); returns the last pass's final
+ * relative CG residual. */
+double slbd_project(slbd_lattice *L, float *U, int CU, int iters, int passes);
 double slbd_features_noalloc(slbd_lattice *L, uint16_t *X, int C);
 void slbd_scratch_init(void);
 /* the 10 moments of the live state on fluid nodes (0 on solid), scaled per

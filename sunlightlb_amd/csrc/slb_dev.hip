@@ -2392,3 +2392,190 @@ extern "C" void slbd_inject(slbd_lattice *L, const slbd_params *P, const float *
   LAUNCHCHECK();
   HIPCHECK(hipStreamSynchronize(L->stream));
 }
+
+/* ---------------------------------------------------------------------- */
+/* Warm start: projection of a velocity field onto the discretely         */
+/* mass-conserving fields of the lattice's fluid nodes.                   */
+/*                                                                        */
+/* Face fluxes F(x, x+e_a) = open(x, x+e_a) (u_a(x) + u_a(x+e_a)) / 2,     */
+/* open = both nodes fluid (no flux through walls), periodic wrap.  The   */
+/* potential phi solves A phi = -div F with the fluid-graph Laplacian     */
+/* A phi(x) = sum over open faces (phi(x) - phi(neighbour)) (conjugate    */
+/* gradients, FP64, scalars kept on the device), so that the fluxes       */
+/* F - (phi(x+e_a) - phi(x)) on open faces carry no divergence.  Each      */
+/* fluid node's velocity is then corrected by the mean of its open face   */
+/* differences along each axis.                                           */
+
+struct ProjNb {
+  long xm[3], xp[3];   /* neighbour indices along x, y, z */
+};
+
+__device__ __forceinline__ ProjNb proj_nb(long n, const Geom &G) {
+  const int i = (int)(n / G.plane), t = (int)(n - (long)i * G.plane);
+  const int j = t / G.esk, k = t - j * G.esk;
+  const AxisOff oi = axis_off(i, G.esi, G.plane);
+  const AxisOff oj = axis_off(j, G.esj, G.esk);
+  const AxisOff ok = axis_off(k, G.esk, 1);
+  ProjNb b;
+  b.xm[0] = n + oi.m; b.xp[0] = n + oi.p;
+  b.xm[1] = n + oj.m; b.xp[1] = n + oj.p;
+  b.xm[2] = n + ok.m; b.xp[2] = n + ok.p;
+  return b;
+}
+
+__device__ __forceinline__ bool proj_fluid(const uint16_t *__restrict__ flags, long n) {
+  return !(flags[n] & FLAG_SOLID);
+}
+
+/* rhs = -div F on fluid nodes (0 on solid); x = 0; r = p = rhs; partial r.r */
+__global__ void __launch_bounds__(256)
+k_proj_init(const uint16_t *__restrict__ flags, const float *__restrict__ U, int CU,
+            double *__restrict__ x, double *__restrict__ r, double *__restrict__ p, Geom G,
+            double *__restrict__ part) {
+  double s[1] = {0.0};
+  for (long n = (long)blockIdx.x * blockDim.x + threadIdx.x; n < G.N;
+       n += (long)gridDim.x * blockDim.x) {
+    double rhs = 0.0;
+    if (proj_fluid(flags, n)) {
+      const ProjNb b = proj_nb(n, G);
+      double div = 0.0;
+      for (int a = 0; a < 3; a++) {
+        const double u0 = U[n * CU + a];
+        if (proj_fluid(flags, b.xp[a])) div += 0.5 * (u0 + (double)U[b.xp[a] * CU + a]);
+        if (proj_fluid(flags, b.xm[a])) div -= 0.5 * ((double)U[b.xm[a] * CU + a] + u0);
+      }
+      rhs = -div;
+    }
+    x[n] = 0.0;
+    r[n] = rhs;
+    p[n] = rhs;
+    s[0] += rhs * rhs;
+  }
+  block_reduce_store<1>(s, part);
+}
+
+/* q = A p; partial p.q */
+__global__ void __launch_bounds__(256)
+k_proj_ap(const uint16_t *__restrict__ flags, const double *__restrict__ p,
+          double *__restrict__ q, Geom G, double *__restrict__ part) {
+  double s[1] = {0.0};
+  for (long n = (long)blockIdx.x * blockDim.x + threadIdx.x; n < G.N;
+       n += (long)gridDim.x * blockDim.x) {
+    double v = 0.0;
+    if (proj_fluid(flags, n)) {
+      const ProjNb b = proj_nb(n, G);
+      const double p0 = p[n];
+      for (int a = 0; a < 3; a++) {
+        if (proj_fluid(flags, b.xp[a])) v += p0 - p[b.xp[a]];
+        if (proj_fluid(flags, b.xm[a])) v += p0 - p[b.xm[a]];
+      }
+      s[0] += p0 * v;
+    }
+    q[n] = v;
+  }
+  block_reduce_store<1>(s, part);
+}
+
+/* Converged: r.r at most PROJ_TOL^2 of its start.  Iterating on past that
+ * point lets p.Ap, a ratio of roundoff-level sums, push the iterate away
+ * again, so a converged iteration changes nothing. */
+#define PROJ_TOL 1e-10
+__device__ __forceinline__ bool proj_done(const double *__restrict__ scal, int rr_slot) {
+  return !(scal[rr_slot] > PROJ_TOL * PROJ_TOL * scal[3]);
+}
+
+/* alpha = rr / pq; x += alpha p; r -= alpha q; partial r.r (new) */
+__global__ void __launch_bounds__(256)
+k_proj_xr(const double *__restrict__ p, const double *__restrict__ q, double *__restrict__ x,
+          double *__restrict__ r, const double *__restrict__ scal, int rr_slot, long N,
+          double *__restrict__ part) {
+  const double rr = scal[rr_slot], pq = scal[2];
+  const double alpha = (pq > 0.0 && !proj_done(scal, rr_slot)) ? rr / pq : 0.0;
+  double s[1] = {0.0};
+  for (long n = (long)blockIdx.x * blockDim.x + threadIdx.x; n < N;
+       n += (long)gridDim.x * blockDim.x) {
+    x[n] += alpha * p[n];
+    const double rn = r[n] - alpha * q[n];
+    r[n] = rn;
+    s[0] += rn * rn;
+  }
+  block_reduce_store<1>(s, part);
+}
+
+/* beta = rr_new / rr_old; p = r + beta p */
+__global__ void __launch_bounds__(256)
+k_proj_p(const double *__restrict__ r, double *__restrict__ p, const double *__restrict__ scal,
+         int old_slot, int new_slot, long N) {
+  const double ro = scal[old_slot], rn = scal[new_slot];
+  const double beta = (ro > 0.0 && !proj_done(scal, old_slot)) ? rn / ro : 0.0;
+  const long n = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n < N) p[n] = r[n] + beta * p[n];
+}
+
+/* u_a(x) -= mean over the open faces along a of the potential difference */
+__global__ void __launch_bounds__(256)
+k_proj_correct(const uint16_t *__restrict__ flags, const double *__restrict__ phi,
+               float *__restrict__ U, int CU, Geom G) {
+  const long n = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= G.N || !proj_fluid(flags, n)) return;
+  const ProjNb b = proj_nb(n, G);
+  const double f0 = phi[n];
+  for (int a = 0; a < 3; a++) {
+    double d = 0.0;
+    int c = 0;
+    if (proj_fluid(flags, b.xp[a])) { d += phi[b.xp[a]] - f0; c++; }
+    if (proj_fluid(flags, b.xm[a])) { d += f0 - phi[b.xm[a]]; c++; }
+    if (c > 0) U[n * CU + a] = (float)((double)U[n * CU + a] - d / c);
+  }
+}
+
+extern "C" double slbd_project(slbd_lattice *L, float *U, int CU, int iters, int passes) {
+  if (iters < 1 || passes < 1 || CU < 3) return 0.0;
+  const long N = L->N;
+  double *w = NULL;
+  HIPCHECK(hipMalloc(&w, sizeof(double) * (4 * (size_t)N + 8)));
+  double *x = w, *r = w + N, *p = w + 2 * N, *q = w + 3 * N, *scal = w + 4 * N;
+  const int nb = red_blocks(N);
+  double h[4] = {0.0, 0.0, 0.0, 0.0};
+  /* each pass: divergence of the current field, CG, correction.  The node
+   * correction (mean of the face differences) removes the face divergence
+   * only approximately, so later passes remove what the earlier ones left.
+   * scal[0], scal[1]: r.r of alternate iterations; scal[2]: p.Ap; scal[3]:
+   * r.r at the start of the pass */
+  for (int pass = 0; pass < passes; pass++) {
+    hipLaunchKernelGGL(k_proj_init, dim3(nb), dim3(256), 0, L->stream, L->flags, U, CU, x, r, p,
+                       L->G, L->part);
+    LAUNCHCHECK();
+    hipLaunchKernelGGL(k_final<1>, dim3(1), dim3(256), 0, L->stream, L->part, nb, scal + 0);
+    hipLaunchKernelGGL(k_final<1>, dim3(1), dim3(256), 0, L->stream, L->part, nb, scal + 3);
+    LAUNCHCHECK();
+    int last = 0;   /* slot of the latest r.r */
+    for (int it = 0; it < iters; it++) {
+      const int cur = it & 1, nxt = cur ^ 1;
+      if (it > 0 && (it & 31) == 0) {
+        /* stop early once converged (a converged iteration is a no-op) */
+        HIPCHECK(hipMemcpyAsync(h, scal, sizeof h, hipMemcpyDeviceToHost, L->stream));
+        HIPCHECK(hipStreamSynchronize(L->stream));
+        if (!(h[cur] > PROJ_TOL * PROJ_TOL * h[3])) break;
+      }
+      last = nxt;
+      hipLaunchKernelGGL(k_proj_ap, dim3(nb), dim3(256), 0, L->stream, L->flags, p, q, L->G,
+                         L->part);
+      hipLaunchKernelGGL(k_final<1>, dim3(1), dim3(256), 0, L->stream, L->part, nb, scal + 2);
+      hipLaunchKernelGGL(k_proj_xr, dim3(nb), dim3(256), 0, L->stream, p, q, x, r, scal, cur, N,
+                         L->part);
+      hipLaunchKernelGGL(k_final<1>, dim3(1), dim3(256), 0, L->stream, L->part, nb, scal + nxt);
+      hipLaunchKernelGGL(k_proj_p, lin_grid(N), dim3(256), 0, L->stream, r, p, scal, cur, nxt, N);
+      LAUNCHCHECK();
+    }
+    hipLaunchKernelGGL(k_proj_correct, lin_grid(N), dim3(256), 0, L->stream, L->flags, x, U, CU,
+                       L->G);
+    LAUNCHCHECK();
+    HIPCHECK(hipMemcpyAsync(h, scal, sizeof h, hipMemcpyDeviceToHost, L->stream));
+    HIPCHECK(hipStreamSynchronize(L->stream));
+    h[2] = h[last];
+  }
+  HIPCHECK(hipFree(w));
+  const double r0 = h[3], rl = h[2];
+  return r0 > 0.0 ? sqrt(rl / r0) : 0.0;
+}

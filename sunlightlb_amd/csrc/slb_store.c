@@ -50,6 +50,65 @@ static void node_rebuild(LBReal *a, LBReal drho, const LBReal j[3], const LBReal
   }
 }
 
+/* The .dms record of one post-propagate node (src/lbstore.c:56-84): the
+ * moments from partial sums in LBReal, each stored as LBStoreReal, with the
+ * diagonal stresses built on the density already rounded to LBStoreReal
+ * (the format's writer keeps drho in an LBStoreReal variable before it
+ * enters s_aa = 2/3 drho - (a_0 + axis pairs) + (pair along a)).  Links 7..10
+ * have c_z = +1, 11..14 c_z = -1; "up"/"dn" are their sums weighted by 1,
+ * c_x, c_y and c_x c_y. */
+static void dms_moments(const LBReal *a, LBStoreReal v[10]) {
+  const LBReal px = a[1] + a[2], py = a[3] + a[4], pz = a[5] + a[6];
+  const LBReal up = a[7] + a[8] + a[9] + a[10], dn = a[11] + a[12] + a[13] + a[14];
+  const LBReal up_x = a[7] - a[8] + a[9] - a[10], dn_x = a[11] - a[12] + a[13] - a[14];
+  const LBReal up_y = a[7] + a[8] - a[9] - a[10], dn_y = a[11] + a[12] - a[13] - a[14];
+  const LBReal up_xy = a[7] - a[8] - a[9] + a[10], dn_xy = a[11] - a[12] - a[13] + a[14];
+  const LBReal rest = a[0] + px + py + pz;
+  const LBStoreReal drho = (LBStoreReal)(rest + (up + dn));
+  const LBReal shift = (2.0 / 3.0) * drho - rest;
+  v[0] = drho;
+  v[1] = (LBStoreReal)(a[1] - a[2] + up_x + dn_x);
+  v[2] = (LBStoreReal)(a[3] - a[4] + up_y + dn_y);
+  v[3] = (LBStoreReal)(a[5] - a[6] + up - dn);
+  v[4] = (LBStoreReal)(shift + px);
+  v[5] = (LBStoreReal)(shift + py);
+  v[6] = (LBStoreReal)(shift + pz);
+  v[7] = (LBStoreReal)(up_xy + dn_xy);
+  v[8] = (LBStoreReal)(up_x - dn_x);
+  v[9] = (LBStoreReal)(up_y - dn_y);
+}
+
+/* One node rebuilt from its .dms record as the format's reader builds it
+ * (src/lbstore.c:138-197): a_l = (w_l/72)[drho + 3 c.j + 9/2 (cc:s - tr/3)],
+ * where the trace and the momentum combinations of the diagonal links are
+ * summed in LBStoreReal (the reader adds its LBStoreReal variables before
+ * promoting them), the rest in LBReal; the momenta were multiplied by
+ * rescale into LBStoreReal first.  weight/add: load_average. */
+static void dms_rebuild(LBReal *a, const LBStoreReal v[10], LBReal weight, int add) {
+  const LBStoreReal jx = v[1], jy = v[2], jz = v[3];
+  const LBStoreReal sxx = v[4], syy = v[5], szz = v[6], sxy = v[7], sxz = v[8], syz = v[9];
+  const LBReal r1 = v[0] / 72.0, r8 = 8.0 * r1;
+  const LBReal third = 8.0 * (1.0 / 24.0), d24 = 1.0 / 24.0;
+  const LBReal tr3 = (LBStoreReal)(sxx + syy + szz) / 3.0;
+  LBReal o[15];
+  o[0] = 2.0 * r8 - tr3;
+  LBReal g = third * jx, h = 0.5 * (sxx - tr3);
+  o[1] = r8 + g + h; o[2] = r8 - g + h;
+  g = third * jy; h = 0.5 * (syy - tr3);
+  o[3] = r8 + g + h; o[4] = r8 - g + h;
+  g = third * jz; h = 0.5 * (szz - tr3);
+  o[5] = r8 + g + h; o[6] = r8 - g + h;
+  g = d24 * (LBStoreReal)(jx + jy + jz); h = 0.125 * (tr3 + sxy + sxz + syz);
+  o[7] = r1 + g + h; o[14] = r1 - g + h;
+  g = d24 * (LBStoreReal)(jx - jy - jz); h = 0.125 * (tr3 - sxy - sxz + syz);
+  o[8] = r1 - g + h; o[13] = r1 + g + h;
+  g = d24 * (LBStoreReal)(jx - jy + jz); h = 0.125 * (tr3 - sxy + sxz - syz);
+  o[9] = r1 + g + h; o[12] = r1 - g + h;
+  g = d24 * (LBStoreReal)(jx + jy - jz); h = 0.125 * (tr3 + sxy - sxz - syz);
+  o[10] = r1 - g + h; o[11] = r1 + g + h;
+  for (int l = 0; l < 15; l++) a[l] = add ? a[l] + weight * o[l] : o[l];
+}
+
 /* lbsave: bbl + propagate, write the 10 moments, collide (src/lbstore.c:36-91). */
 void lbsave(LBSys lb, char *savefile) {
   FILE *fp = fopen(savefile, "wb");
@@ -62,11 +121,8 @@ void lbsave(LBSys lb, char *savefile) {
   propagate(lb);
   slb_host_pops(lb);
   for (long n = 0; n < lb->nnodes; n++) {
-    LBReal d, j[3], s[6];
-    node_moments(lb->a + 15 * n, &d, j, s);
-    LBStoreReal v[10] = {(LBStoreReal)d, (LBStoreReal)j[0], (LBStoreReal)j[1], (LBStoreReal)j[2],
-                         (LBStoreReal)s[0], (LBStoreReal)s[1], (LBStoreReal)s[2],
-                         (LBStoreReal)s[3], (LBStoreReal)s[4], (LBStoreReal)s[5]};
+    LBStoreReal v[10];
+    dms_moments(lb->a + 15 * n, v);
     fwrite(v, sizeof(LBStoreReal), 10, fp);
   }
   fclose(fp);
@@ -93,9 +149,8 @@ void lbload(LBSys lb, char *loadfile) {
   for (long n = 0; n < lb->nnodes; n++) {
     LBStoreReal v[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     if (fread(v, sizeof(LBStoreReal), 10, fp) != 10) { /* short file: zeros, as fread leaves */ }
-    LBReal d = v[0], j[3] = {v[1], v[2], v[3]}, s[6] = {v[4], v[5], v[6], v[7], v[8], v[9]};
-    for (int q = 0; q < 3; q++) j[q] = (LBStoreReal)(lb->rescale * (LBStoreReal)j[q]);
-    node_rebuild(lb->a + 15 * n, d, j, s, 0.5, lb->load_average);
+    for (int q = 1; q <= 3; q++) v[q] = (LBStoreReal)(lb->rescale * v[q]);
+    dms_rebuild(lb->a + 15 * n, v, 0.5, lb->load_average);
   }
   fclose(fp);
   if (lb->skip_solid)

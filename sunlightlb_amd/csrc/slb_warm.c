@@ -86,6 +86,8 @@ struct slb_warmnet {
   float *loss;
   float *wc_eval;
   int mode;
+  int proj_iters;       /* divergence projection of the prediction (0: off): */
+  int proj_passes;      /* CG iterations per pass, passes */
   /* training set */
   int nsamp, cap;
   wn_sample *samp;
@@ -624,7 +626,24 @@ void slb_warm_start(slb_warmnet *net, LBSys lb) {
   target_scales(lb, &P, ell, sc);
   forward(net);
   slbn_sync();
+  if (net->proj_iters > 0 && net->proj_passes > 0)
+    slbd_project(PRIV(lb)->dev, net->outf, WN_C, net->proj_iters, net->proj_passes);
   inject(lb, &P, sc, net->outf, WN_C, net->mode);
+}
+
+void slb_warm_set_projection(slb_warmnet *net, int iters, int passes) {
+  net->proj_iters = iters > 0 ? iters : 0;
+  net->proj_passes = passes > 0 ? passes : 0;
+}
+void slb_warm_get_projection(const slb_warmnet *net, int *iters, int *passes) {
+  if (iters) *iters = net->proj_iters;
+  if (passes) *passes = net->proj_passes;
+}
+
+double slb_warm_project(LBSys lb, float *m, int channels, int iters, int passes) {
+  slb_dev_geom(lb);
+  slbn_sync();
+  return slbd_project(PRIV(lb)->dev, m, channels, iters, passes);
 }
 
 void slb_warm_predict(slb_warmnet *net, LBSys lb, float *m) {

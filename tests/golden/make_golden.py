@@ -8,6 +8,7 @@ Run here, where the reference sources exist:
     make -C oracle ref && python tests/golden/make_golden.py
     python tests/golden/make_golden.py packs      # only the workload-size tables
     python tests/golden/make_golden.py pack256    # only the 256^3 permeability pack
+    python tests/golden/make_golden.py checkpoint # only the checkpoint / velocity-field cases
 
 Every lattice is non-cubic so that a transposed index cannot pass.  Body
 forces use the reference's current API (zerobf + addbf on fluid nodes).
@@ -587,6 +588,132 @@ def case_packs(R):
     print("wrote packs")
 
 
+CKPT = dict(dims=(12, 10, 14), tau=0.8, sphere=(3.3, 5.2, 4.7, 6.9), wall=("y", 0),
+            bf=(1e-4, 2e-5, -3e-5), sphere_vel=(0.011, 0.0, -0.002), sphere_ang=(0.0, 0.001, -0.003),
+            pre_steps=28, rescale=2.0)
+
+
+def ckpt_lattice(R, c):
+    """The checkpoint case: Stokes flow past a sphere beside a wall, a body
+    force, object velocities set (no moving walls: they only reach the
+    object file)."""
+    h = R.new(*c["dims"])
+    st = h.contents
+    st.tau, st.stokes_flow = c["tau"], 1
+    R.L.insertwall(h, c["wall"][0].encode(), c["wall"][1])
+    sph = R.L.insertsph(h, *c["sphere"])
+    sph.contents.velx, sph.contents.vely, sph.contents.velz = c["sphere_vel"]
+    sph.contents.angx, sph.contents.angy, sph.contents.angz = c["sphere_ang"]
+    R.L.zerobf(h)
+    R.L.addbf(h, *c["bf"])
+    return h
+
+
+def case_checkpoint(R):
+    """Checkpoint files (src/lbstore.c:36-91 lbsave, :523-547 lbsaveobj) of
+    the reference after pre_steps steps, measure() (object forces in the
+    file) and one more step -- what a USR2 signal during that step writes
+    (src/lbsignal.c:49-59) -- and what the reference's lbload (:102-218) and
+    lbloadobj (:555-586, also with rescale) read back from them."""
+    import tempfile
+    L = R.L
+    vp, sp = C.c_void_p, C.c_char_p
+    for n, (r, a) in {"lbsaveobj": (None, [vp, sp]), "lbloadobj": (None, [vp, sp])}.items():
+        getattr(L, n).restype = r
+        getattr(L, n).argtypes = a
+    c = CKPT
+    h = ckpt_lattice(R, c)
+    for _ in range(c["pre_steps"]):
+        L.lbstep(h)
+    L.measure(h)
+    L.lbstep(h)
+    out = {"meta": np.array(json.dumps(c))}
+    N = h.contents.nnodes
+    cwd = os.getcwd()
+    with tempfile.TemporaryDirectory() as d:
+        os.chdir(d)
+        try:
+            L.lbsave(h, b"ck.dms")
+            L.lbsaveobj(C.cast(h, vp), b"ck.obj")
+            out["a_after_save"] = R.arr(h.contents.a, 15 * N).astype(np.float64).reshape(N, 15).copy()
+            out["dms"] = np.frombuffer(open("ck.dms", "rb").read(), dtype=np.uint8).copy()
+            out["obj"] = np.array(open("ck.obj").read())
+            # read back into fresh lattices of the same geometry
+            h2 = ckpt_lattice(R, c)
+            L.syncobjs(h2)
+            L.lbload(h2, b"ck.dms")
+            out["a_loaded"] = R.arr(h2.contents.a, 15 * N).astype(np.float64).reshape(N, 15).copy()
+            for o in R.objects(h2):
+                o.velx = o.vely = o.velz = o.angx = o.angy = o.angz = 0.0
+            L.lbloadobj(C.cast(h2, vp), b"ck.obj")
+            out["loadobj_rows"] = objrows(R, h2)
+            h3 = ckpt_lattice(R, c)
+            h3.contents.rescale = c["rescale"]
+            L.lbloadobj(C.cast(h3, vp), b"ck.obj")
+            out["loadobj_rescaled_rows"] = objrows(R, h3)
+            out["rescale_after"] = np.array(h3.contents.rescale)
+        finally:
+            os.chdir(cwd)
+    np.savez_compressed(os.path.join(OUT, "checkpoint.npz"), **out)
+    print("wrote checkpoint")
+
+
+VF = dict(dims=(11, 13, 9), tau=0.7, sphere=(3.1, 5.5, 6.0, 4.2), steps=21, prefac=(1.0, 0.5),
+          vset=[[5, 6, 4, 0.01, -0.02, 0.005], [0, 0, 0, 0.3, 0.3, 0.3], [11, 2, 2, 1.0, 1.0, 1.0],
+                [6, 5, 3, -0.004, 0.0, 0.007]])
+
+
+def case_vfields(R):
+    """The velocity-field calls (src/lbmeasure.c:110-166 vzero/vgrab,
+    src/lbobjs.c:623-643 vset) on a Navier-Stokes lattice with a sphere and
+    a non-uniform body force: vgrab after collide (correction -b/2) and after
+    bbl + propagate (+b/2), accumulating with different prefactors; vset on
+    solid, fluid and out-of-range nodes; vzero clearing the fluid nodes."""
+    L = R.L
+    for n, (r, a) in {"vzero": (None, [C.c_void_p]),
+                      "vgrab": (None, [C.c_void_p, C.c_double]),
+                      "vset": (None, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_double,
+                                      C.c_double, C.c_double])}.items():
+        getattr(L, n).restype = r
+        getattr(L, n).argtypes = a
+    c = VF
+    h = R.new(*c["dims"])
+    st = h.contents
+    st.tau, st.stokes_flow = c["tau"], 0
+    L.insertsph(h, *c["sphere"])
+    L.zerobf(h)
+    N = st.nnodes
+    rng = np.random.default_rng(31)
+    objid = R.objid(h)
+    bx, by, bz = (R.arr(getattr(st, f), N) for f in ("bx", "by", "bz"))
+    for arr, sc in ((bx, 2e-4), (by, -1e-4), (bz, 5e-5)):
+        arr[:] = np.where(objid < 0, sc * (1.0 + rng.random(N)), 0.0)
+    out = {"meta": np.array(json.dumps(c)), "bx": bx.copy(), "by": by.copy(), "bz": bz.copy()}
+    for _ in range(c["steps"]):
+        L.lbstep(h)
+    hv = C.cast(h, C.c_void_p)
+
+    def fields():
+        return np.stack([R.arr(getattr(st, f), N).astype(np.float64) for f in ("ux", "uy", "uz")])
+
+    for q, (i, j, k, vx, vy, vz) in enumerate(c["vset"]):
+        L.vset(hv, int(i), int(j), int(k), vx, vy, vz)
+    out["u_vset"] = fields()
+    L.vzero(hv)
+    L.vgrab(hv, c["prefac"][0])
+    out["u_grab_collide"] = fields()
+    L.bbl(h)
+    L.propagate(h)
+    L.vgrab(hv, c["prefac"][1])
+    out["u_grab_propagate"] = fields()
+    L.collide(h)
+    L.vzero(hv)
+    out["u_zero"] = fields()
+    out["a"] = R.arr(st.a, 15 * N).astype(np.float64).reshape(N, 15).copy()
+    np.savez_compressed(os.path.join(OUT, "vfields.npz"), **out)
+    print("wrote vfields")
+
+
 MASK64 = (1 << 64) - 1
 
 
@@ -719,6 +846,10 @@ def main():
     if sys.argv[1:] == ["pack256"]:
         case_pack256(R)
         return
+    if sys.argv[1:] == ["checkpoint"]:
+        case_checkpoint(R)
+        case_vfields(R)
+        return
     R32 = RefLib(32)
     case_tables(R)
     case_static(R, "static_stokes", (12, 10, 14), [[3.3, 5.2, 4.7, 6.9]],
@@ -734,6 +865,8 @@ def main():
     case_sphere_c(R)
     case_images(R)
     case_diagnostics(R)
+    case_checkpoint(R)
+    case_vfields(R)
     case_packs(R)
 
 

@@ -175,26 +175,6 @@ def test_batch_graph_replay_after_odd_run(slb):
         slb.slb_batch_free(b2)
 
 
-def test_batch_autotune_after_odd_run_then_replay(slb):
-    """The autotuner times candidate shapes and returns the batch to rest
-    with buffer 0 live; after an odd run before it, the replayed step loop
-    must still start from the live buffer."""
-    b, vels = make_batch(slb)
-    try:
-        slb.slb_batch_run(b, 33)
-        assert slb.slb_batch_autotune(b, None) == 0
-        slb.slb_batch_run(b, 40)
-        slb.slb_batch_measure(b)
-        box = BOXES[1]
-        N = int(np.prod(box["dims"]))
-        a = np.zeros(15 * N)
-        slb.slb_batch_get_pops(b, 1, 2, a.ctypes.data_as(C.POINTER(C.c_double)))
-        a_ref, _, _ = run_alone(box, vels[1][2], 40)
-        assert rel_l2(a, a_ref) < 1e-12
-    finally:
-        slb.slb_batch_free(b)
-
-
 @pytest.mark.parametrize("cfg", [(256, 1, 0), (128, 4, 8)])
 def test_batch_against_reference_fixture(slb, cfg):
     """The batched kernel against the reference library directly (no

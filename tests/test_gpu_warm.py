@@ -581,3 +581,44 @@ def test_conv_lds_kernel_is_bit_identical_and_matches_oracle(shape, cin, cout, t
         ref = np.maximum(onn.conv3d(x, w, bias.astype(np.float64)), 0.0)
         err = np.abs(outs[0][1].astype(np.float64) - ref).max() / np.abs(ref).max()
         assert err < 1e-5
+
+
+@pytest.mark.parametrize("passes", [1, 3])
+def test_projection_matches_oracle(passes):
+    """slb_warm_project (CG on the fluid-graph Laplacian, FP64, then the
+    node correction; FP32 tensor) against the oracle's exact solve of each
+    pass, on a pack with a smooth divergent field plus noise; the channels
+    beyond 0..2 and the solid nodes are left untouched."""
+    import sunlightlb_amd as s
+    L = lib()
+    dims = (12, 14, 16)
+    lat = s.Lattice(*dims, precision=32)
+    for sp in ((3.5, 5.0, 6.0, 7.0), (3.0, 10.0, 2.0, 12.0), (2.5, 1.0, 11.0, 3.0)):
+        L.insertsph(lat.h, *sp)
+    L.syncobjs(lat.h)
+    solid = lat.solid().reshape(dims)
+    rng = np.random.default_rng(8)
+    i, j, k = np.meshgrid(*[np.arange(n) for n in dims], indexing="ij")
+    u = np.stack([np.cos(2 * np.pi * (i / 12 + 2 * k / 16)), np.sin(2 * np.pi * j / 14),
+                  0.5 * np.cos(2 * np.pi * (i / 12 - j / 14))], -1)
+    u += 0.1 * rng.standard_normal(u.shape)
+    u[solid] = 0.0
+    C4 = 4
+    m = np.zeros(dims + (C4,), np.float32)
+    m[..., :3] = u
+    m[..., 3] = 7.0
+    u = m[..., :3].astype(np.float64)          # the FP32 values the GPU starts from
+    d = Dev(host=m)
+    res = L.slb_warm_project(lat.h, d.p, C4, 600, passes)
+    got = d.get(np.float32, dims + (C4,))
+    d.free()
+    lat.free()
+    ref, _ = onn.project(u, solid, passes=passes)
+    print(f"projection passes {passes}: CG residual {res:.2e}, "
+          f"rel L2 vs oracle {np.linalg.norm(got[..., :3] - ref) / np.linalg.norm(ref):.2e}")
+    assert res < 1e-8
+    assert np.linalg.norm(got[..., :3] - ref) < 1e-5 * np.linalg.norm(ref)
+    assert (got[..., 3] == 7.0).all()
+    assert (got[solid][:, :3] == 0.0).all()
+    d0 = np.linalg.norm(onn.face_divergence(u, solid))
+    assert np.linalg.norm(onn.face_divergence(got[..., :3].astype(np.float64), solid)) < 0.3 * d0
