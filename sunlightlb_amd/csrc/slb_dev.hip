@@ -155,6 +155,108 @@ k_step(const T *__restrict__ src, T *__restrict__ dst,
   }
 }
 
+/* The step with the k pulls staged in LDS.  For rows whose length esk
+ * divides the 512-thread block (64, 128, 256 or 512 nodes), a block holds
+ * whole rows, and the population a_l pulled from x - c_l lies in row
+ * (i - cx, j - cy) at k - cz (periodic within the row).  The 10 links with
+ * cz != 0 are loaded aligned -- each thread loads the source row's value at
+ * its own k -- into LDS, and each thread reads its value at k - cz there, so
+ * every global load of a wavefront covers whole 64-value stretches instead
+ * of one shifted by an element (two 128-byte lines per wave and population
+ * instead of three).  Links with cz = 0 and bounced links (a node's own
+ * opposite population) load from global memory as in k_step.  Same values,
+ * same arithmetic: bit-identical to k_step. */
+template <typename T, bool MOVING, bool FORCE, int BF, typename I>
+__global__ void __launch_bounds__(512)
+k_step_lds(const T *__restrict__ src, T *__restrict__ dst,
+           const uint16_t *__restrict__ flags,
+           const double *__restrict__ ux, const double *__restrict__ uy,
+           const double *__restrict__ uz,
+           const T *__restrict__ bx, const T *__restrict__ by, const T *__restrict__ bz,
+           double *__restrict__ fx, double *__restrict__ fy, double *__restrict__ fz,
+           Geom G, PhysArgs P) {
+  __shared__ T row[10][512];
+  const int t = blockIdx.x * 512 + threadIdx.x;     /* plane % 512 == 0: all in range */
+  const int i = blockIdx.y;
+  const int kmask = G.esk - 1;
+  const int j = t / G.esk;
+  const int k = t & kmask;
+  const I x = (I)i * (I)G.plane + (I)t;
+  const AxisOffT<I> oi = axis_off_t<I>(i, G.esi, (I)G.plane);
+  const AxisOffT<I> oj = axis_off_t<I>(j, G.esj, (I)G.esk);
+  const AxisOffT<I> ok = axis_off_t<I>(k, G.esk, (I)1);
+  const AxisOffT<I> o0 = {0, 0};
+  const unsigned fl = flags[x];
+  const bool solid = (fl & FLAG_SOLID) != 0;
+  T v[15];
+  /* stage: the source row of each cz != 0 link, aligned (no k shift) */
+  Links<5, 15>::run([&](auto LL) {
+    constexpr int l = decltype(LL)::value;
+    row[l - 5][threadIdx.x] = src[pidx_t<I>(l, x + nb_off<l, -1>(oi, oj, o0))];
+  });
+  v[0] = src[pidx_t<I>(0, x)];
+  Links<1, 5>::run([&](auto LL) {
+    constexpr int l = decltype(LL)::value;
+    const bool bb = (fl >> (l - 1)) & 1u;
+    v[l] = src[bb ? pidx_t<I>(OPP(l), x) : pidx_t<I>(l, x + nb_off<l, -1>(oi, oj, ok))];
+  });
+  T own[15];
+  Links<5, 15>::run([&](auto LL) {
+    constexpr int l = decltype(LL)::value;
+    own[l] = ((fl >> (l - 1)) & 1u) ? src[pidx_t<I>(OPP(l), x)] : T(0);
+  });
+  __syncthreads();
+  const int rbase = threadIdx.x - k;      /* first thread of this row in the block */
+  Links<5, 15>::run([&](auto LL) {
+    constexpr int l = decltype(LL)::value;
+    const T pulled = row[l - 5][rbase + ((k - CZ(l)) & kmask)];
+    v[l] = ((fl >> (l - 1)) & 1u) ? own[l] : pulled;
+    if (FORCE) own[l] = pulled;   /* the force needs a_l(x - c_l) as well */
+  });
+  if ((MOVING || FORCE) && (fl & FLAG_LINKS)) {
+    double Fx = 0.0, Fy = 0.0, Fz = 0.0;
+    Links<1, 15>::run([&](auto LL) {
+      constexpr int l = decltype(LL)::value;
+      if ((fl >> (l - 1)) & 1u) {
+        double cu = 0.0;
+        if (MOVING) {
+          const I s = solid ? x : x + nb_off<l, -1>(oi, oj, ok);
+          cu = (double)WL(l) * ((CX(l) ? CX(l) * ux[s] : 0.0) + (CY(l) ? CY(l) * uy[s] : 0.0) +
+                                (CZ(l) ? CZ(l) * uz[s] : 0.0));
+        }
+        if (FORCE && solid) {
+          const double a = (double)v[l];
+          const double b = CZ(l) ? (double)own[l]
+                                 : (double)src[pidx_t<I>(l, x + nb_off<l, -1>(oi, oj, ok))];
+          const double g = -2.0 * (a - b + cu);
+          if (CX(l)) Fx += CX(l) * g;
+          if (CY(l)) Fy += CY(l) * g;
+          if (CZ(l)) Fz += CZ(l) * g;
+        }
+        if (MOVING) v[l] = (T)((double)v[l] + cu);
+      }
+    });
+    if (FORCE && solid) {
+      fx[x] += Fx; fy[x] += Fy; fz[x] += Fz;
+    }
+  }
+  if (!(P.skip_solid && solid)) {
+    T jbx = T(0), jby = T(0), jbz = T(0);
+    if (BF != 0 && !solid) {
+      jbx = T(P.bfx); jby = T(P.bfy); jbz = T(P.bfz);
+      if (BF == 2) { jbx += bx[x]; jby += by[x]; jbz += bz[x]; }
+    }
+    collide_node<T>(v, jbx, jby, jbz, P);
+  }
+  if (G.nt_store) {
+#pragma unroll
+    for (int l = 0; l < 15; l++) __builtin_nontemporal_store(v[l], &dst[pidx_t<I>(l, x)]);
+  } else {
+#pragma unroll
+    for (int l = 0; l < 15; l++) dst[pidx_t<I>(l, x)] = v[l];
+  }
+}
+
 /* ---------------------------------------------------------------------- */
 /* Separate phases (bbl, propagate, collide) on the live buffer.           */
 
@@ -786,6 +888,7 @@ struct slbd_lattice {
   uint32_t *mpf;
   hipEvent_t ev0, ev1;
   int sweep_nt_mid;         /* two-step sweep: non-temporal window stores */
+  int step_lds;             /* step with the k pulls staged in LDS (k_step_lds) */
 };
 
 struct slbd_mp {
@@ -865,6 +968,15 @@ extern "C" slbd_lattice *slbd_create(int esi, int esj, int esk, int prec) {
   }
   L->G.xcd_map = step_order(L->G);
   L->sweep_nt_mid = getenv("SLB_SWEEP_NT") ? atoi(getenv("SLB_SWEEP_NT")) : 0;
+  {
+    /* k_step_lds needs whole rows per 512-thread block */
+    /* measured: 362.5 -> 359.1 us per FP32 step at 256^3 (729 against 733 us
+     * FP64), bit-identical */
+    const int want = getenv("SLB_STEP_LDS") ? atoi(getenv("SLB_STEP_LDS")) : 1;
+    const int e = esk;
+    L->step_lds = want && (e == 64 || e == 128 || e == 256 || e == 512) &&
+                  (L->G.plane % 512) == 0;
+  }
   {
     /* MP kernel: 256-thread blocks over (row block, plane) */
     const long nb = (long)((L->G.plane + 255) / 256) * esi;
@@ -1363,6 +1475,19 @@ static void launch_step_t(slbd_lattice *L, const PhysArgs &A, const void *srcv, 
   const T *src = (const T *)srcv;
   T *dst = (T *)dstv;
   const SweepMap M = {i0, sring, dring};
+  if (L->step_lds && i0 == 0 && np == L->esi && sring == 0 && dring == 0) {
+    const dim3 g2((unsigned)(G.plane / 512), (unsigned)np);
+    if (L->idx32)
+      hipLaunchKernelGGL((k_step_lds<T, MOVING, FORCE, BF, int>), g2, dim3(512), 0, L->stream,
+                         src, dst, L->flags, L->ux, L->uy, L->uz, (const T *)L->bx,
+                         (const T *)L->by, (const T *)L->bz, L->fx, L->fy, L->fz, G, A);
+    else
+      hipLaunchKernelGGL((k_step_lds<T, MOVING, FORCE, BF, long>), g2, dim3(512), 0, L->stream,
+                         src, dst, L->flags, L->ux, L->uy, L->uz, (const T *)L->bx,
+                         (const T *)L->by, (const T *)L->bz, L->fx, L->fy, L->fz, G, A);
+    LAUNCHCHECK();
+    return;
+  }
   const dim3 grid((unsigned)((G.plane + G.step_tpb - 1) / G.step_tpb), (unsigned)np);
   if (L->idx32)
     hipLaunchKernelGGL((k_step<T, MOVING, FORCE, BF, int>), grid, dim3(G.step_tpb), 0,

@@ -104,3 +104,41 @@ def test_sweep_fp64_matches_oracle(oracle, monkeypatch):
     oracle.lbstep(ao, mask, objid, dims, bb, 0.7, 0, 1, u=u, nsteps=24)
     assert rel_l2(lat.pops(), ao) < 1e-12
     lat.free()
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+@pytest.mark.parametrize("dims", [(12, 16, 64), (9, 8, 128), (6, 6, 256)])
+def test_lds_staged_step_bit_identical(prec, dims, monkeypatch):
+    """k_step_lds (the k pulls staged in LDS; rows of 64-512 nodes) against
+    k_step: populations after odd and even step counts and the node forces
+    of measure(), moving walls, Navier-Stokes, per-node body force."""
+    import sunlightlb_amd as s
+
+    def run(lds):
+        monkeypatch.setenv("SLB_STEP_LDS", lds)
+        lat = s.Lattice(*dims, precision=prec)
+        L, h = lat.L, lat.h
+        lat.s.tau, lat.s.stokes_flow, lat.s.objects_moving = 0.7, 0, 1
+        sph = L.insertsph(h, 3.2, dims[0] / 2, dims[1] / 2, 5.5)
+        sph2 = L.insertsph(h, 2.5, 1.0, 2.0, dims[2] - 2.0)     # wraps in k
+        sph.contents.velx, sph.contents.angz = 0.01, 0.002
+        sph2.contents.velz = -0.02
+        L.vspread(h)
+        L.zerobf(h)
+        lat.sync()
+        rng = np.random.default_rng(4)
+        solid = lat.solid()
+        for nm, sc in (("bx", 2e-4), ("by", 5e-5), ("bz", 1e-4)):
+            s.view(getattr(lat.s, nm), lat.N)[:] = np.where(solid, 0.0, sc * (1 + rng.random(lat.N)))
+        L.slb_sync_device(h)
+        L.slb_run(h, 17)
+        a = lat.pops()
+        L.measure(h)
+        f = np.stack([lat.field(n) for n in ("fx", "fy", "fz")])
+        out = (a, lat.pops(), f)
+        lat.free()
+        return out
+
+    ref, got = run("0"), run("1")
+    for r, g in zip(ref, got):
+        assert np.array_equal(r, g)
