@@ -904,7 +904,7 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / PEAK_HBM_GBS, 4),
             "traffic": traffic_bytes(n, args.precision),
-            "kernel": "k_step (fused pull step)",
+            "kernel": "k_step_lds (fused pull step, k pulls staged in LDS)",
             "algorithmic_bytes_per_node": bytes_per_lup,
             "device_copy_GBs": round(copy_gbs, 1),
         },
