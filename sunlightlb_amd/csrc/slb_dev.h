@@ -71,6 +71,18 @@ void slbd_scale_pops(slbd_lattice *L, double s);
 /* LB step: nsteps fused bbl+propagate+collide; force != 0 accumulates the
  * bblforce node force (fx,fy,fz) during the bounce-back. */
 void slbd_step(slbd_lattice *L, const slbd_params *P, int nsteps, int force);
+/* Random sequential addition, candidate pre-filter on the device (see
+ * k_rsa_filter in slb_dev.hip): a cell grid of the accepted spheres (host
+ * layout: heads per cell, next per sphere, centres), and for trials t0 ..
+ * t0+m-1 of the splitmix64 stream from seed, pass[q] = 0 when the trial's
+ * centre clearly overlaps an uploaded sphere (squared minimum-image
+ * distance below d2min). */
+typedef struct slbd_rsa slbd_rsa;
+slbd_rsa *slbd_rsa_create(const double L[3], const double cs[3], const int nc[3]);
+void slbd_rsa_update(slbd_rsa *R, const int *head, const int *next, const double *xyz, int n);
+void slbd_rsa_filter(slbd_rsa *R, uint64_t seed, long t0, int m, double d2min, uint8_t *pass);
+void slbd_rsa_destroy(slbd_rsa *R);
+
 /* Launch plan of the two-step sweep over esi planes in chunks of P (see
  * sweep2 in slb_dev.hip): rows of {phase, i0, np, sring, dring}, phase 1 =
  * step t from the live buffer into the plane window of the other buffer,

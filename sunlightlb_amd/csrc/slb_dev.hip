@@ -2704,3 +2704,140 @@ extern "C" double slbd_project(slbd_lattice *L, float *U, int CU, int iters, int
   const double r0 = h[3], rl = h[2];
   return r0 > 0.0 ? sqrt(rl / r0) : 0.0;
 }
+
+/* ---------------------------------------------------------------------- */
+/* Random sequential addition: candidate pre-filter.                       */
+/*                                                                        */
+/* Trial t of the splitmix64 stream from `seed` draws its centre from the  */
+/* stream's outputs 3t+1, 3t+2, 3t+3 (x, y, z), and output k is            */
+/* mix(seed + k * 0x9E3779B97F4A7C15): any trial's centre is computed      */
+/* directly.  A batch of candidates is tested in parallel against the      */
+/* spheres accepted before the batch (cell grid, minimum image); a         */
+/* candidate that overlaps one of them is rejected by the sequential       */
+/* process too, since that sphere was accepted first.  The host then runs  */
+/* its exact sequential test on the survivors only, in trial order, so the */
+/* accepted spheres are the sequential process's.  The device test drops   */
+/* only clear overlaps (squared distance below (1 - 1e-9) of the limit):   */
+/* its rounding never decides a case the host would decide otherwise.      */
+
+__host__ __device__ inline uint64_t rsa_mix(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  return z ^ (z >> 31);
+}
+
+__global__ void __launch_bounds__(256)
+k_rsa_filter(uint64_t seed, long t0, int m, const int *__restrict__ head,
+             const int *__restrict__ next, const double *__restrict__ xyz, int nc0, int nc1,
+             int nc2, double cs0, double cs1, double cs2, double L0, double L1, double L2,
+             double d2lim, uint8_t *__restrict__ pass) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= m) return;
+  const uint64_t g = 0x9E3779B97F4A7C15ULL;
+  const uint64_t k0 = 3ULL * (uint64_t)(t0 + q);
+  const double s53 = 1.0 / 9007199254740992.0;
+  const double L[3] = {L0, L1, L2}, cs[3] = {cs0, cs1, cs2};
+  const int nc[3] = {nc0, nc1, nc2};
+  double c[3];
+  int ci[3];
+  for (int d = 0; d < 3; d++) {
+    c[d] = (double)(rsa_mix(seed + (k0 + 1 + d) * g) >> 11) * s53 * L[d];
+    int v = (int)floor(c[d] / cs[d]);
+    ci[d] = v >= nc[d] ? nc[d] - 1 : (v < 0 ? 0 : v);
+  }
+  uint8_t ok = 1;
+  for (int a = -1; a <= 1 && ok; a++)
+    for (int b = -1; b <= 1 && ok; b++)
+      for (int e = -1; e <= 1 && ok; e++) {
+        const int x = (ci[0] + a + nc0) % nc0, y = (ci[1] + b + nc1) % nc1,
+                  z = (ci[2] + e + nc2) % nc2;
+        for (int s = head[(x * nc1 + y) * nc2 + z]; s >= 0; s = next[s]) {
+          double r2 = 0.0;
+          for (int d = 0; d < 3; d++) {
+            double dx = c[d] - xyz[3 * s + d];
+            dx -= L[d] * floor(dx / L[d] + 0.5);
+            r2 += dx * dx;
+          }
+          if (r2 < d2lim) { ok = 0; break; }
+        }
+      }
+  pass[q] = ok;
+}
+
+struct slbd_rsa {
+  int nc[3], ncell, cap, n;
+  double cs[3], L[3];
+  int *head, *next;
+  double *xyz;
+  uint8_t *pass;
+  int pass_cap;
+  hipStream_t stream;
+};
+
+extern "C" slbd_rsa *slbd_rsa_create(const double L[3], const double cs[3], const int nc[3]) {
+  slbd_rsa *R = (slbd_rsa *)calloc(1, sizeof(slbd_rsa));
+  for (int d = 0; d < 3; d++) { R->nc[d] = nc[d]; R->cs[d] = cs[d]; R->L[d] = L[d]; }
+  R->ncell = nc[0] * nc[1] * nc[2];
+  R->cap = 4096;
+  HIPCHECK(hipStreamCreateWithFlags(&R->stream, hipStreamNonBlocking));
+  HIPCHECK(hipMalloc(&R->head, sizeof(int) * R->ncell));
+  HIPCHECK(hipMalloc(&R->next, sizeof(int) * R->cap));
+  HIPCHECK(hipMalloc(&R->xyz, sizeof(double) * 3 * R->cap));
+  HIPCHECK(hipMemsetAsync(R->head, 0xff, sizeof(int) * R->ncell, R->stream));
+  return R;
+}
+
+/* The accepted spheres so far: the host's cell heads, and spheres n0.. with
+ * their list links and centres. */
+extern "C" void slbd_rsa_update(slbd_rsa *R, const int *head, const int *next,
+                                const double *xyz, int n) {
+  if (n > R->cap) {
+    int cap = R->cap;
+    while (cap < n) cap *= 2;
+    int *nx;
+    double *xy;
+    HIPCHECK(hipMalloc(&nx, sizeof(int) * cap));
+    HIPCHECK(hipMalloc(&xy, sizeof(double) * 3 * cap));
+    HIPCHECK(hipStreamSynchronize(R->stream));
+    HIPCHECK(hipFree(R->next));
+    HIPCHECK(hipFree(R->xyz));
+    R->next = nx; R->xyz = xy; R->cap = cap;
+    R->n = 0;   /* everything is uploaded again below */
+  }
+  if (n > R->n) {
+    HIPCHECK(hipMemcpyAsync(R->next + R->n, next + R->n, sizeof(int) * (n - R->n),
+                            hipMemcpyHostToDevice, R->stream));
+    HIPCHECK(hipMemcpyAsync(R->xyz + 3L * R->n, xyz + 3L * R->n, sizeof(double) * 3 * (n - R->n),
+                            hipMemcpyHostToDevice, R->stream));
+  }
+  HIPCHECK(hipMemcpyAsync(R->head, head, sizeof(int) * R->ncell, hipMemcpyHostToDevice, R->stream));
+  HIPCHECK(hipStreamSynchronize(R->stream));
+  R->n = n;
+}
+
+/* pass[q] = 0 when trial t0 + q clearly overlaps a sphere uploaded so far. */
+extern "C" void slbd_rsa_filter(slbd_rsa *R, uint64_t seed, long t0, int m, double d2min,
+                                uint8_t *pass) {
+  if (m > R->pass_cap) {
+    if (R->pass) HIPCHECK(hipFree(R->pass));
+    HIPCHECK(hipMalloc(&R->pass, m));
+    R->pass_cap = m;
+  }
+  hipLaunchKernelGGL(k_rsa_filter, dim3((m + 255) / 256), dim3(256), 0, R->stream, seed, t0, m,
+                     R->head, R->next, R->xyz, R->nc[0], R->nc[1], R->nc[2], R->cs[0], R->cs[1],
+                     R->cs[2], R->L[0], R->L[1], R->L[2], d2min * (1.0 - 1e-9), R->pass);
+  LAUNCHCHECK();
+  HIPCHECK(hipMemcpyAsync(pass, R->pass, m, hipMemcpyDeviceToHost, R->stream));
+  HIPCHECK(hipStreamSynchronize(R->stream));
+}
+
+extern "C" void slbd_rsa_destroy(slbd_rsa *R) {
+  if (!R) return;
+  HIPCHECK(hipStreamSynchronize(R->stream));
+  HIPCHECK(hipFree(R->head));
+  HIPCHECK(hipFree(R->next));
+  HIPCHECK(hipFree(R->xyz));
+  if (R->pass) HIPCHECK(hipFree(R->pass));
+  HIPCHECK(hipStreamDestroy(R->stream));
+  free(R);
+}

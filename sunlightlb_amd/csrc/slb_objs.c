@@ -522,6 +522,17 @@ static uint64_t splitmix64(uint64_t *s) {
 }
 static double unit01(uint64_t *s) { return (splitmix64(s) >> 11) * (1.0 / 9007199254740992.0); }
 
+/* Centre of trial t: the stream's draws 3t+1, 3t+2, 3t+3 times the box edges,
+ * x then y then z (the state after k draws is seed + k times the splitmix64
+ * increment, so trial t needs no earlier draws). */
+static void rsa_centre(uint64_t seed, long t, const double *L, double *c) {
+  uint64_t st = seed + 3ULL * (uint64_t)t * 0x9E3779B97F4A7C15ULL;
+  const double x = unit01(&st) * L[0];
+  const double y = unit01(&st) * L[1];
+  const double z = unit01(&st) * L[2];
+  c[0] = x; c[1] = y; c[2] = z;
+}
+
 /* Cell grid for the minimum-image overlap test. */
 typedef struct {
   int nc[3];
@@ -591,36 +602,57 @@ int slb_rsa_pack(LBSys lb, double R, double phi, unsigned long long seed, long m
   }
   g.head = (int *)slb_xmalloc(sizeof(int) * ncell);
   for (int c = 0; c < ncell; c++) g.head[c] = -1;
-  uint64_t st = (uint64_t)seed;
   long target = (long)ceil(phi * (double)lb->nnodes);
   long solid = 0;
   for (long n = 0; n < lb->nnodes; n++) solid += lb->obj[n] != NULL;
   int placed = 0;
   int saved = slb_verbosity;
   slb_verbosity &= ~OBJECT_DETAILS;
-  for (long t = 0; t < maxtrials && solid < target; t++) {
-    double c[3] = {unit01(&st) * g.L[0], unit01(&st) * g.L[1], unit01(&st) * g.L[2]};
-    if (rsa_overlaps(&g, c, 4.0 * R * R)) continue;
-    rsa_add(&g, c);
-    for (int a = -1; a <= 1; a++)
-      for (int b = -1; b <= 1; b++)
-        for (int e = -1; e <= 1; e++) {
-          double o[3] = {c[0] + a * g.L[0], c[1] + b * g.L[1], c[2] + e * g.L[2]};
-          if (o[0] + R < 0.0 || o[0] - R > g.L[0] || o[1] + R < 0.0 || o[1] - R > g.L[1] ||
-              o[2] + R < 0.0 || o[2] - R > g.L[2])
-            continue;
-          LBObj p = addobj(lb);
-          zeroobj(p);
-          int ov = 0;
-          int ns = raster_sphere(lb, p, R, o[0], o[1], o[2], &ov);
-          if (ns == 0) { drop_head(lb, p); lb->next_id--; continue; }
-          p->posx = o[0]; p->posy = o[1]; p->posz = o[2];
-          solid += ns;
-          lb->changed_objects = 1;
-        }
-    if (centres && placed < maxcentres) memcpy(centres + 3 * placed, c, sizeof(double) * 3);
-    placed++;
+  /* Trials in batches: the device drops the candidates that clearly overlap
+   * a sphere accepted before the batch (late in a pack, nearly all of them);
+   * the rest get the exact sequential test here, in trial order. */
+  slbd_rsa *D = slbd_rsa_create(g.L, g.cs, g.nc);
+  int m = 1024, uploaded = -1;
+  uint8_t *pass = (uint8_t *)slb_xmalloc(1 << 20);
+  long t = 0;
+  while (t < maxtrials && solid < target) {
+    const int mm = (maxtrials - t < m) ? (int)(maxtrials - t) : m;
+    if (uploaded != g.n) { slbd_rsa_update(D, g.head, g.next, g.xyz, g.n); uploaded = g.n; }
+    slbd_rsa_filter(D, (uint64_t)seed, t, mm, 4.0 * R * R, pass);
+    int q, npass = 0;
+    for (q = 0; q < mm && solid < target; q++) {
+      if (!pass[q]) continue;
+      npass++;
+      double c[3];
+      rsa_centre((uint64_t)seed, t + q, g.L, c);
+      if (rsa_overlaps(&g, c, 4.0 * R * R)) continue;
+      rsa_add(&g, c);
+      for (int a = -1; a <= 1; a++)
+        for (int b = -1; b <= 1; b++)
+          for (int e = -1; e <= 1; e++) {
+            double o[3] = {c[0] + a * g.L[0], c[1] + b * g.L[1], c[2] + e * g.L[2]};
+            if (o[0] + R < 0.0 || o[0] - R > g.L[0] || o[1] + R < 0.0 || o[1] - R > g.L[1] ||
+                o[2] + R < 0.0 || o[2] - R > g.L[2])
+              continue;
+            LBObj p = addobj(lb);
+            zeroobj(p);
+            int ov = 0;
+            int ns = raster_sphere(lb, p, R, o[0], o[1], o[2], &ov);
+            if (ns == 0) { drop_head(lb, p); lb->next_id--; continue; }
+            p->posx = o[0]; p->posy = o[1]; p->posz = o[2];
+            solid += ns;
+            lb->changed_objects = 1;
+          }
+      if (centres && placed < maxcentres) memcpy(centres + 3 * placed, c, sizeof(double) * 3);
+      placed++;
+    }
+    t += q;
+    /* batch size: a few hundred survivors per batch at most */
+    if (npass < 64 && m < (1 << 20)) m *= 2;
+    else if (npass > 512 && m > 256) m /= 2;
   }
+  free(pass);
+  slbd_rsa_destroy(D);
   slb_verbosity = saved;
   slb_obj_changed(lb);
   if (phi_out) *phi_out = (double)solid / (double)lb->nnodes;
@@ -668,12 +700,12 @@ int slb_pack_map(int esi, int esj, int esk, double R, double phi, unsigned long 
     for (int c = 0; c < ncell; c++) g.head[c] = -1;
   }
   memset(solid, 0, (size_t)N);
-  uint64_t st = (uint64_t)seed;
   const long target = (long)ceil(phi * (double)N);
   long nsolid = 0;
   int placed = 0;
   for (long t = 0; t < maxtrials && nsolid < target; t++) {
-    double c[3] = {unit01(&st) * g.L[0], unit01(&st) * g.L[1], unit01(&st) * g.L[2]};
+    double c[3];
+    rsa_centre((uint64_t)seed, t, g.L, c);
     if (!overlap) {
       if (rsa_overlaps(&g, c, 4.0 * R * R)) continue;
       rsa_add(&g, c);

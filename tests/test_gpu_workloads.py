@@ -336,3 +336,72 @@ def test_batch512_boxes_match_live_oracle(oracle):
                     f"force {rel_l2(frc, res['frc']):.1e}")
     finally:
         L.slb_batch_free(b)
+
+
+def test_mp128_peclet_scaling_closed_form():
+    """The Peclet scaling of the passive-scalar workload (Pe = U 2R / D, U the
+    interstitial mean speed, D = (1 - alpha)/6): without adsorption a
+    uniform tracer's mean displacement grows as U t, so after scaling the
+    flow to Pe 10 it is 10 D / (2R) per step (measured 2.7e-4 off after 1000
+    steps: the tracer drifts with the populations' own momentum, U is the
+    force-corrected mean; bound 1e-3)."""
+    L = lib()
+    h, mp = mp_setup(L, 64, pads=0.0)
+    L.slb_mp_track_moments(mp, 0)
+    L.slb_mp_run(mp, 1000)
+    s1 = (C.c_double * 6)()
+    L.slb_mp_sums(mp, s1)
+    L.slb_destroy(h)
+    mean = s1[3] / s1[0]
+    expect = 10.0 * (1.0 / 6.0) / 12.0 * 1000
+    log(f"mp128 Pe 10, no adsorption: mean displacement {mean:.6f}, U t = {expect:.6f}")
+    assert abs(mean - expect) < 1e-3 * expect
+
+
+# ---------------------------------------------------------- learned warm start
+
+WEIGHTS = os.path.join(os.path.dirname(GOLDEN), "..", "sunlightlb_amd", "weights",
+                       "warm_unet128.bin")
+
+
+def test_committed_weights_solve_unseen_packs():
+    """The committed network (bench.py --warm-only --warm-save, trained on
+    2000 GPU-solved 128^3 packs) loaded with slb_warm_load: two unseen 128^3
+    packs solved with slb_warm_solve (start policy and fallback) reach
+    residual 1e-6 with the cold solve's permeability within 1e-5; without
+    the safeguards the warm start reaches 1e-4 in fewer steps than the cold
+    solve (the gain the network gives at moderate tolerances)."""
+    import numpy as np
+    L = lib()
+    net = L.slb_warm_load(os.path.abspath(WEIGHTS).encode())
+    assert net, "weights file missing or refused"
+    params = bench.warm_params(np.random.default_rng(SEED + 1234), 2)
+    ratios = []
+    for (solid, phi), p in zip(bench.warm_maps(L, 128, params, 4), params):
+        res = {}
+        for mode in ("cold", "warm", "warm_trace"):
+            h = bench.warm_lattice(L, 128, solid, 1e-6)
+            if mode == "cold":
+                steps, hist, r = bench.solve_loop(L, h, 1e-6, 100, 40000)
+            elif mode == "warm":
+                rr, uw = C.c_double(0.0), C.c_int(0)
+                steps = L.slb_warm_solve(net, h, 1e-6, 100, 40000, C.byref(rr), C.byref(uw))
+                r = rr.value
+            else:
+                L.slb_warm_set_mode(net, 1)
+                L.slb_warm_start(net, h)
+                steps, hist, r = bench.solve_loop(L, h, 1e-4, 100, 40000)
+            L.measure(h)
+            res[mode] = (steps, r, h.contents.vmeanx)
+            if mode == "cold":
+                cold_hist = hist
+            L.slb_destroy(h)
+        cold_to_1e4 = next((i + 1) * 100 for i, x in enumerate(cold_hist) if 0 <= x < 1e-4)
+        log(f"pack R {p[0]:.2f} phi {phi:.3f}: cold {res['cold'][0]} steps, warm solve "
+            f"{res['warm'][0]}, to 1e-4 cold {cold_to_1e4} warm {res['warm_trace'][0]}")
+        assert 0 <= res["warm"][1] < 1e-6 and 0 <= res["cold"][1] < 1e-6
+        kc, kw = res["cold"][2], res["warm"][2]
+        assert abs(kw - kc) < 1e-5 * abs(kc)
+        ratios.append(cold_to_1e4 / res["warm_trace"][0])
+    assert min(ratios) > 1.0, ratios
+    L.slb_warm_free(net)
