@@ -714,6 +714,78 @@ def case_vfields(R):
     print("wrote vfields")
 
 
+def mobility2_replay(L, new, objects, vel_setter, step, measure, vspread, removeobjs, insert):
+    """examples/mobility2.pl: lateral mobility of a sphere above a wall by
+    relaxing to zero torque on the fly -- for each height, 20 rounds of 100
+    steps + measure(), the sphere translating at vx = 0.1 with angular
+    velocity ay from the secant update of the torque; vspread after each
+    velocity change (the current library's moving-wall step reads it).
+    Returns D/D0 per height and the last (ay, ty) per height."""
+    import math
+    esx, esy, esz, nsteps, a_hyd, a_nom = 48, 32, 32, 2000, 4.1, 4.3
+    lb = new(esx, esy, esz)
+    eta = 72.0 * (0.95 - 0.5) / 3.0
+    out = []
+    for height in (5.0, 6.0, 7.0, 8.0, 9.0):
+        sph = insert(lb, a_nom, esx / 2 - 0.5, esy / 2 - 0.5, height + 0.5)
+        vx, ntries = 0.1, 20
+        num = nsteps // ntries
+        ay0 = ay1 = ty0 = ty1 = 0.0
+        for j in range(ntries):
+            if j == 0:
+                ay = 0.0
+            elif j == 1:
+                ay = 0.01
+            else:
+                ay = ay1 - ty1 * (ay1 - ay0) / (ty1 - ty0)
+            vel_setter(sph, vx, ay)
+            vspread(lb)
+            for _ in range(num):
+                step(lb)
+            fx, ty = measure(lb, sph)
+            if j == 0:
+                ay0, ty0 = ay, ty
+            elif j == 1:
+                ay1, ty1 = ay, ty
+            else:
+                ay0, ay1, ty0, ty1 = ay1, ay, ty1, ty
+        removeobjs(lb)
+        m = vx / fx
+        out.append(dict(height=height, a_by_h=a_hyd / height, M=m,
+                        DbyD0=-6 * math.pi * eta * a_hyd * m, ay=ay, ty=ty))
+    return out
+
+
+def case_mobility2(R):
+    L = R.L
+
+    def new(esx, esy, esz):
+        h = R.new(esx, esy, esz)
+        st = h.contents
+        st.objects_moving, st.vel_corrected, st.skip_solid, st.stokes_flow = 1, 1, 0, 1
+        st.tau = 0.95
+        return h
+
+    def insert(h, a, x, y, z):
+        L.insertwall(h, b"z", 0)
+        return L.insertsph(h, a, x, y, z)
+
+    def setv(sph, vx, ay):
+        sph.contents.velx, sph.contents.angy = vx, ay
+
+    def meas(h, sph):
+        L.measure(h)
+        return sph.contents.frcx, sph.contents.trqy
+
+    L.removeobjs.argtypes = [C.c_void_p]
+    L.removeobjs.restype = None
+    out = mobility2_replay(L, new, None, setv, L.lbstep, meas, L.vspread,
+                           lambda h: L.removeobjs(C.cast(h, C.c_void_p)), insert)
+    with open(os.path.join(OUT, "mobility2.json"), "w") as f:
+        json.dump(out, f, indent=1)
+    print("wrote mobility2", [round(r["DbyD0"], 4) for r in out])
+
+
 MASK64 = (1 << 64) - 1
 
 
@@ -846,6 +918,9 @@ def main():
     if sys.argv[1:] == ["pack256"]:
         case_pack256(R)
         return
+    if sys.argv[1:] == ["mobility2"]:
+        case_mobility2(R)
+        return
     if sys.argv[1:] == ["checkpoint"]:
         case_checkpoint(R)
         case_vfields(R)
@@ -867,6 +942,7 @@ def main():
     case_diagnostics(R)
     case_checkpoint(R)
     case_vfields(R)
+    case_mobility2(R)
     case_packs(R)
 
 

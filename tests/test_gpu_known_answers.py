@@ -158,3 +158,47 @@ def test_sphere_above_wall_mobility():
     lat.free()
     for d, r in zip(out, (0.522, 0.618, 0.680, 0.724, 0.759)):
         assert near(d, r, 3), (d, r)
+
+
+def test_sphere_above_wall_zero_torque_mobility():
+    """examples/mobility2.pl:32-36 -- the same mobility by relaxing the
+    sphere's angular velocity to zero torque on the fly (secant updates every
+    100 steps, 20 rounds per height).  As for mobility1.pl, the script's
+    table came from an older library; the current reference library, with
+    vspread after each velocity change, prints .522 .618 .680 .725 .759
+    (published: .522 .620 .682 .727 .762), and that replay
+    (tests/golden/mobility2.json, make_golden.py) is the bar: every D/D0 to
+    its printed digit and to 1e-9 relative in FP64."""
+    import json
+    import sys
+    sys.path.insert(0, GOLDEN)
+    from make_golden import mobility2_replay
+    ref = json.load(open(os.path.join(GOLDEN, "mobility2.json")))
+    import sunlightlb_amd as s
+    L = s.lib()
+    lats = []
+
+    def new(esx, esy, esz):
+        lat = lattice(esx, esy, esz)
+        lat.s.objects_moving, lat.s.vel_corrected, lat.s.skip_solid = 1, 1, 0
+        lat.s.stokes_flow, lat.s.tau = 1, 0.95
+        lats.append(lat)
+        return lat.h
+
+    def insert(h, a, x, y, z):
+        L.insertwall(h, b"z", 0)
+        return L.insertsph(h, a, x, y, z)
+
+    def setv(sph, vx, ay):
+        sph.contents.velx, sph.contents.angy = vx, ay
+
+    def meas(h, sph):
+        L.measure(h)
+        return sph.contents.frcx, sph.contents.trqy
+
+    got = mobility2_replay(L, new, None, setv, L.lbstep, meas, L.vspread, L.removeobjs, insert)
+    lats[0].free()
+    for g, r in zip(got, ref):
+        print(f"a_hyd/h {g['a_by_h']:8.3f}  D/D0 {g['DbyD0']:8.3f} (reference {r['DbyD0']:.6f})")
+        assert f"{g['DbyD0']:.3f}" == f"{r['DbyD0']:.3f}"
+        assert abs(g["DbyD0"] - r["DbyD0"]) < 1e-9 * abs(r["DbyD0"])
