@@ -131,11 +131,51 @@ def cpu_mp_baseline(L, h, n, steps):
                        f"steps after one untimed step, {dt:.1f} s")
 
 
-def cpu_batch_baseline(boxes, steps, u0, tau, nbox=16):
-    """Time the reference's lbstep (FP32 build, one core) on the first boxes of
-    the batch, sphere 0 and its images moving along x (one right-hand side
-    per box; the other five cost the same).  The boxes are independent, so
-    a host with C cores runs C of them at once."""
+def _ref_moving_box(R, Lb, sph, u0, tau):
+    """One box of the batch in the reference library: sphere 0 and its
+    periodic images moving along x, the other spheres fixed."""
+    rh = R.L.newlb(Lb, Lb, Lb, b"null")
+    st = rh.contents
+    st.tau, st.stokes_flow, st.objects_moving = tau, 1, 1
+    for q in range(len(sph) - 1, -1, -1):
+        Rq, c = sph[q][0], sph[q][1:]
+        for a in (-1, 0, 1):
+            for b_ in (-1, 0, 1):
+                for e in (-1, 0, 1):
+                    o = (c[0] + a * Lb, c[1] + b_ * Lb, c[2] + e * Lb)
+                    if all(-Rq < o[d] < Lb + Rq for d in range(3)):
+                        p = R.L.insertsph(rh, Rq, *o)
+                        if p and q == 0:
+                            p.contents.velx = u0
+    R.L.vspread(rh)
+    R.L.lbstep(rh)
+    return rh
+
+
+def _ref_boxes_worker(boxes, steps, u0, tau, barrier, out):
+    """One process of the aggregate leg: set its boxes up, wait for every
+    other process, then step each box in turn; reports node updates and the
+    monotonic-clock interval of the stepping."""
+    from oracle import RefLib
+    R = RefLib(32)
+    hs = [(_ref_moving_box(R, Lb, sph, u0, tau), Lb) for (Lb, sph) in boxes]
+    barrier.wait()
+    t0 = time.monotonic()
+    for rh, _ in hs:
+        for _ in range(steps):
+            R.L.lbstep(rh)
+    out.put((sum(Lb ** 3 for _, Lb in hs) * steps, t0, time.monotonic()))
+
+
+def cpu_batch_baseline(boxes, steps, u0, tau, nbox=16, per_proc=8, max_procs=16):
+    """The reference's lbstep (FP32 build) on boxes of the batch, sphere 0 and
+    its images moving along x (one right-hand side per box; the other five
+    cost the same).  Serial leg: one core steps the first `nbox` boxes in turn.
+    Aggregate leg: the boxes are independent, so one process per host core
+    (at most `max_procs`) steps `per_proc` boxes of its own, all at once; the
+    aggregate is every process's node updates over the interval from the
+    first start to the last finish."""
+    import multiprocessing as mp
     from oracle import RefLib
     try:
         R = RefLib(32)
@@ -144,31 +184,40 @@ def cpu_batch_baseline(boxes, steps, u0, tau, nbox=16):
         return None
     nodes, dt = 0, 0.0
     for (Lb, sph) in boxes[:nbox]:
-        rh = R.L.newlb(Lb, Lb, Lb, b"null")
-        st = rh.contents
-        st.tau, st.stokes_flow, st.objects_moving = tau, 1, 1
-        for q in range(len(sph) - 1, -1, -1):
-            Rq, c = sph[q][0], sph[q][1:]
-            for a in (-1, 0, 1):
-                for b_ in (-1, 0, 1):
-                    for e in (-1, 0, 1):
-                        o = (c[0] + a * Lb, c[1] + b_ * Lb, c[2] + e * Lb)
-                        if all(-Rq < o[d] < Lb + Rq for d in range(3)):
-                            p = R.L.insertsph(rh, Rq, *o)
-                            if p and q == 0:
-                                p.contents.velx = u0
-        R.L.vspread(rh)
-        R.L.lbstep(rh)
+        rh = _ref_moving_box(R, Lb, sph, u0, tau)
         t0 = time.perf_counter()
         for _ in range(steps):
             R.L.lbstep(rh)
         dt += time.perf_counter() - t0
         nodes += Lb ** 3
-    return dict(value=round(nodes * steps / dt / 1e6, 3), unit="MLUPS", cores=1,
-                host_cores=host_cores(), kind="reference",
-                sample=f"reference lbstep with moving walls (-D_SINGLE, serial) on the first "
-                       f"{min(nbox, len(boxes))} boxes of the batch, one right-hand side each, "
-                       f"{steps} steps per box, {dt:.1f} s")
+    res = dict(value=round(nodes * steps / dt / 1e6, 3), unit="MLUPS", cores=1,
+               host_cores=host_cores(), kind="reference",
+               sample=f"reference lbstep with moving walls (-D_SINGLE, serial) on the first "
+                      f"{min(nbox, len(boxes))} boxes of the batch, one right-hand side each, "
+                      f"{steps} steps per box, {dt:.1f} s")
+    nproc = max(1, min(host_cores(), max_procs, len(boxes) // per_proc))
+    # fresh interpreters (spawn): the parent holds a GPU context, the
+    # workers touch only the host library
+    ctx = mp.get_context("spawn")
+    barrier, out = ctx.Barrier(nproc), ctx.Queue()
+    procs = [ctx.Process(target=_ref_boxes_worker,
+                         args=(boxes[w * per_proc:(w + 1) * per_proc], steps, u0, tau,
+                               barrier, out)) for w in range(nproc)]
+    for p in procs:
+        p.start()
+    try:
+        got = [out.get(timeout=600) for _ in procs]
+    finally:
+        for p in procs:
+            p.join(timeout=60)
+    upd = sum(g[0] for g in got)
+    span = max(g[2] for g in got) - min(g[1] for g in got)
+    res["aggregate"] = dict(
+        value=round(upd / span / 1e6, 3), unit="MLUPS", cores=nproc,
+        sample=f"{nproc} processes at once, {per_proc} boxes each "
+               f"({nproc * per_proc} distinct boxes of the batch), {steps} steps per box, "
+               f"{span:.1f} s from first start to last finish")
+    return res
 
 
 def traffic_bytes(n, precision):

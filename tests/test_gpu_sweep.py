@@ -107,7 +107,7 @@ def test_sweep_fp64_matches_oracle(oracle, monkeypatch):
 
 
 @pytest.mark.parametrize("prec", [64, 32])
-@pytest.mark.parametrize("dims", [(12, 16, 64), (9, 8, 128), (6, 6, 256)])
+@pytest.mark.parametrize("dims", [(12, 16, 64), (9, 8, 128), (6, 6, 256), (1, 1, 512), (3, 2, 256)])
 def test_lds_staged_step_bit_identical(prec, dims, monkeypatch):
     """k_step_lds (the k pulls staged in LDS; rows of 64-512 nodes) against
     k_step: populations after odd and even step counts and the node forces
