@@ -18,14 +18,15 @@
 
 typedef char slb_lbreal_is_double[(sizeof(LBReal) == sizeof(double)) ? 1 : -1];
 
+/* Zero-filled arrays; release with slb_big_free. */
 LBReal *slb_ralloc(long n) {
-  LBReal *p = (LBReal *)calloc((size_t)(n > 0 ? n : 1), sizeof(LBReal));
+  LBReal *p = (LBReal *)slb_big_calloc(n, sizeof(LBReal));
   if (p == NULL) error("No room for LBReal array");
   return p;
 }
 
 int *slb_ialloc(long n) {
-  int *p = (int *)calloc((size_t)(n > 0 ? n : 1), sizeof(int));
+  int *p = (int *)slb_big_calloc(n, sizeof(int));
   if (p == NULL) error("No room for int array");
   return p;
 }
@@ -61,7 +62,7 @@ LBSys newlb(int reqesi, int reqesj, int reqesk, char *name) {
   long N = (long)lb->nnodes;
   lb->a = slb_ralloc(15 * N);
   lb->mask = slb_ialloc(N);
-  lb->obj = (LBObj *)calloc((size_t)(N > 0 ? N : 1), sizeof(LBObj));
+  lb->obj = (LBObj *)slb_big_calloc(N, sizeof(LBObj));
   if (lb->obj == NULL) error("No room for LBObj array\n");
   lb->ux = slb_ralloc(N); lb->uy = slb_ralloc(N); lb->uz = slb_ralloc(N);
   lb->fx = slb_ralloc(N); lb->fy = slb_ralloc(N); lb->fz = slb_ralloc(N);
@@ -210,6 +211,36 @@ static uint64_t ptr_hash(const void *p) {
   return x;
 }
 
+typedef struct {
+  LBObj *obj;
+  int32_t *objidx;
+  LBObj *keys;
+  int *vals;
+  int cap;
+  long nfluid[SLB_PAR_MAX];
+  int bad[SLB_PAR_MAX];
+} objidx_pass;
+
+static void objidx_range(void *c, long lo, long hi, int tid) {
+  objidx_pass *J = (objidx_pass *)c;
+  LBObj last = NULL;
+  int lastv = -1;
+  long nf = 0;
+  J->bad[tid] = 0;
+  for (long n = lo; n < hi; n++) {
+    LBObj p = J->obj[n];
+    if (p == NULL) { J->objidx[n] = -1; nf++; continue; }
+    if (p != last) {
+      uint64_t h = ptr_hash(p) & (J->cap - 1);
+      while (J->keys[h] != NULL && J->keys[h] != p) h = (h + 1) & (J->cap - 1);
+      if (J->keys[h] != p) { J->bad[tid] = 1; break; }
+      last = p; lastv = J->vals[h];
+    }
+    J->objidx[n] = lastv;
+  }
+  J->nfluid[tid] = nf;
+}
+
 static void build_objidx(LBSys lb) {
   slb_lbpriv *S = PRIV(lb);
   slb_objtab(lb);
@@ -223,22 +254,16 @@ static void build_objidx(LBSys lb) {
     keys[h] = S->objlist[o];
     vals[h] = o;
   }
-  long N = lb->nnodes;
-  LBObj last = NULL;
-  int lastv = -1, nfluid = 0;
-  for (long n = 0; n < N; n++) {
-    LBObj p = lb->obj[n];
-    if (p == NULL) { S->objidx[n] = -1; nfluid++; continue; }
-    if (p != last) {
-      uint64_t h = ptr_hash(p) & (cap - 1);
-      while (keys[h] != NULL && keys[h] != p) h = (h + 1) & (cap - 1);
-      if (keys[h] != p) error("Node refers to an object that is not in the object list");
-      last = p; lastv = vals[h];
-    }
-    S->objidx[n] = lastv;
-  }
+  objidx_pass J;
+  J.obj = lb->obj; J.objidx = S->objidx; J.keys = keys; J.vals = vals; J.cap = cap;
+  const int nr = slb_par_for(lb->nnodes, objidx_range, &J);
   free(keys); free(vals);
-  S->nfluid = nfluid;
+  long nfluid = 0;
+  for (int t = 0; t < nr; t++) {
+    if (J.bad[t]) error("Node refers to an object that is not in the object list");
+    nfluid += J.nfluid[t];
+  }
+  S->nfluid = (int)nfluid;
   slbd_put_objidx(S->dev, S->objidx);
   S->objidx_dirty = 0;
   S->flags_stale = 1;
@@ -273,15 +298,42 @@ void slb_dev_geom(LBSys lb) {
 /* Body force analysis: the force on a fluid node is b[n] + (dpdx,dpdy,dpdz).
  * When b is the same on every fluid node it travels as a kernel argument,
  * otherwise the per-node arrays are uploaded. */
-static void analyse_bf(LBSys lb) {
-  slb_lbpriv *S = PRIV(lb);
-  long N = lb->nnodes;
+typedef struct {
+  LBSys lb;
+  int have[SLB_PAR_MAX], uniform[SLB_PAR_MAX];
+  double u[SLB_PAR_MAX][3];
+} bf_pass;
+
+/* per range: its first fluid node's force and whether every later fluid
+ * node of the range equals it */
+static void bf_range(void *c, long lo, long hi, int tid) {
+  bf_pass *J = (bf_pass *)c;
+  LBSys lb = J->lb;
   int have = 0, uniform = 1;
   double u0 = 0, u1 = 0, u2 = 0;
-  for (long n = 0; n < N; n++) {
+  for (long n = lo; n < hi; n++) {
     if (lb->obj[n] != NULL) continue;
     if (!have) { u0 = lb->bx[n]; u1 = lb->by[n]; u2 = lb->bz[n]; have = 1; continue; }
     if (lb->bx[n] != u0 || lb->by[n] != u1 || lb->bz[n] != u2) { uniform = 0; break; }
+  }
+  J->have[tid] = have; J->uniform[tid] = uniform;
+  J->u[tid][0] = u0; J->u[tid][1] = u1; J->u[tid][2] = u2;
+}
+
+static void analyse_bf(LBSys lb) {
+  slb_lbpriv *S = PRIV(lb);
+  bf_pass J;
+  J.lb = lb;
+  const int nr = slb_par_for(lb->nnodes, bf_range, &J);
+  /* the same verdict and value as one scan in node order: the first fluid
+   * node's force, uniform when every range is and starts with that force */
+  int have = 0, uniform = 1;
+  double u0 = 0, u1 = 0, u2 = 0;
+  for (int t = 0; t < nr && uniform; t++) {
+    if (!J.have[t]) continue;
+    if (!J.uniform[t]) { uniform = 0; break; }
+    if (!have) { u0 = J.u[t][0]; u1 = J.u[t][1]; u2 = J.u[t][2]; have = 1; continue; }
+    if (J.u[t][0] != u0 || J.u[t][1] != u1 || J.u[t][2] != u2) uniform = 0;
   }
   S->bf_uniform = uniform;
   if (uniform) {
@@ -415,17 +467,17 @@ void makemask(LBSys lb) {
   slb_lbpriv *S = PRIV(lb);
   build_objidx(lb);
   slbd_makemask(S->dev);
-  slbd_get_mask(S->dev, lb->mask);
-  S->mask_loc = LOC_SYNC;
+  S->mask_loc = LOC_DEV;      /* the host copy follows when a host reader asks */
   slbd_build_flags(S->dev);
   S->flags_stale = 0;
   if (slb_verbosity & TRACE_ROUTINE_EXITS) printf("LB %i : makemask done\n", lb->step);
 }
 
 int countfluid(LBSys lb) {
-  long N = lb->nnodes;
-  int count = 0;
-  for (long n = 0; n < N; n++) if (lb->obj[n] == NULL) count++;
+  slb_lbpriv *S = PRIV(lb);
+  /* the fluid count of obj[] as the dense index was built from it */
+  if (S->objidx_dirty) build_objidx(lb);
+  const int count = S->nfluid;
   if (slb_verbosity & TRACE_ROUTINE_EXITS)
     printf("LB %i : countfluid done, = %i\n", lb->step, count);
   return count;
@@ -439,9 +491,17 @@ void syncobjs(LBSys lb) {
 }
 
 int countmask(LBSys lb, char *type) {
-  slb_host_mask(lb);
+  slb_lbpriv *S = PRIV(lb);
   long N = lb->nnodes;
   int count = 0;
+  /* the device copy of the mask is current unless the host wrote it */
+  if (S->mask_loc != LOC_HOST && (type[0] == 'a' || type[0] == 'l' || type[0] == 'f' ||
+                                  type[0] == 's')) {
+    const int faces = (type[0] == 'f' || type[0] == 's');
+    count = (int)slbd_countmask(S->dev, faces, lb->w);
+    if (slb_verbosity & TRACE_ROUTINE_EXITS) printf("LB %i : countmask done\n", lb->step);
+    return count;
+  }
   switch (type[0]) {
   case 'a': case 'l':
     for (long n = 0; n < N; n++) {
@@ -475,20 +535,41 @@ void vspread(LBSys lb) {
 }
 
 /* Body force on fluid nodes, src/lbobjs.c:593-619. */
+/* A store of +0.0 only where the value differs in its bits: the result is
+ * the same, and pages of the force arrays that were never written (newlb's
+ * zeroed allocation) are not touched. */
+static inline void set_zero(LBReal *p) {
+  static const LBReal z = 0.0;
+  if (memcmp(p, &z, sizeof z) != 0) *p = 0.0;
+}
+static void zerobf_range(void *c, long lo, long hi, int tid) {
+  LBSys lb = (LBSys)c;
+  (void)tid;
+  for (long n = lo; n < hi; n++)
+    if (lb->obj[n] == NULL) { set_zero(&lb->bx[n]); set_zero(&lb->by[n]); set_zero(&lb->bz[n]); }
+}
+
 void zerobf(LBSys lb) {
-  long N = lb->nnodes;
-  for (long n = 0; n < N; n++)
-    if (lb->obj[n] == NULL) lb->bx[n] = lb->by[n] = lb->bz[n] = 0.0;
+  slb_par_for(lb->nnodes, zerobf_range, lb);
   slb_bf_changed(lb);
   if (slb_verbosity & TRACE_ROUTINE_EXITS) printf("LB %i : zerobf done\n", lb->step);
 }
 
+typedef struct {
+  LBSys lb;
+  LBReal d[3];
+} addbf_pass;
+static void addbf_range(void *c, long lo, long hi, int tid) {
+  addbf_pass *J = (addbf_pass *)c;
+  LBSys lb = J->lb;
+  (void)tid;
+  for (long n = lo; n < hi; n++)
+    if (lb->obj[n] == NULL) { lb->bx[n] += J->d[0]; lb->by[n] += J->d[1]; lb->bz[n] += J->d[2]; }
+}
+
 void addbf(LBSys lb, double dpdx, double dpdy, double dpdz) {
-  long N = lb->nnodes;
-  for (long n = 0; n < N; n++)
-    if (lb->obj[n] == NULL) {
-      lb->bx[n] += (LBReal)dpdx; lb->by[n] += (LBReal)dpdy; lb->bz[n] += (LBReal)dpdz;
-    }
+  addbf_pass J = {lb, {(LBReal)dpdx, (LBReal)dpdy, (LBReal)dpdz}};
+  slb_par_for(lb->nnodes, addbf_range, &J);
   slb_bf_changed(lb);
   if (slb_verbosity & TRACE_ROUTINE_EXITS) printf("LB %i : zerobf done\n", lb->step);
 }
@@ -765,8 +846,8 @@ void slb_destroy(LBSys lb) {
   for (slb_mpwrap *m = S->mps; m;) {
     slb_mpwrap *next = m->priv.next;
     slbd_mp_destroy(m->priv.dev);
-    free(m->pub.p);
-    free(m->pub.newp);
+    slb_big_free(m->pub.p);
+    slb_big_free(m->pub.newp);
     free(m->pub.name);
     free(m);
     m = next;
@@ -777,11 +858,12 @@ void slb_destroy(LBSys lb) {
     free(p);
     p = q;
   }
-  free(lb->c); free(lb->w); free(lb->a); free(lb->mask); free(lb->obj);
-  free(lb->ux); free(lb->uy); free(lb->uz);
-  free(lb->fx); free(lb->fy); free(lb->fz);
-  free(lb->bx); free(lb->by); free(lb->bz);
-  free(lb->rho); free(lb->name);
+  slb_big_free(lb->c); slb_big_free(lb->w); slb_big_free(lb->a); slb_big_free(lb->mask);
+  slb_big_free(lb->obj);
+  slb_big_free(lb->ux); slb_big_free(lb->uy); slb_big_free(lb->uz);
+  slb_big_free(lb->fx); slb_big_free(lb->fy); slb_big_free(lb->fz);
+  slb_big_free(lb->bx); slb_big_free(lb->by); slb_big_free(lb->bz);
+  slb_big_free(lb->rho); free(lb->name);
   free(S->objlist); free(S->objtab); free(S->objidx);
   free((slb_lbwrap *)lb);
 }

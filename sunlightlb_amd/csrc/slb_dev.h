@@ -56,6 +56,7 @@ void slbd_zero_pops(slbd_lattice *L);
 void slbd_put_objidx(slbd_lattice *L, const int32_t *objidx);
 void slbd_get_objidx(slbd_lattice *L, int32_t *objidx);
 void slbd_makemask(slbd_lattice *L);                 /* mask from objidx (bit-exact) */
+long slbd_countmask(slbd_lattice *L, int faces, const int *w); /* countmask on the device mask */
 void slbd_put_mask(slbd_lattice *L, const int32_t *mask);
 void slbd_get_mask(slbd_lattice *L, int32_t *mask);
 void slbd_build_flags(slbd_lattice *L);              /* pull flags from mask + objidx */

@@ -365,6 +365,38 @@ k_makemask(const int32_t *__restrict__ objidx, int32_t *__restrict__ mask, Geom 
   mask[x] = m;
 }
 
+/* countmask (src/lbobjs.c:671-714) over the device mask: per node with
+ * mask > 0, the weights w_l of its link bits (links) or the number of its
+ * face bits l = 1..6 (faces); exact integer sums. */
+struct CountW {
+  int w[15];
+};
+__global__ void __launch_bounds__(256)
+k_countmask(const int32_t *__restrict__ mask, long N, int faces, CountW W,
+            unsigned long long *__restrict__ out) {
+  __shared__ unsigned red[256];
+  const long x = (long)blockIdx.x * 256 + threadIdx.x;
+  unsigned v = 0;
+  if (x < N) {
+    const int m = mask[x];
+    if (m > 0) {
+      if (faces) {
+        v = (unsigned)__popc((unsigned)m & 0x3fu);
+      } else {
+        for (int l = 1; l < 15; l++)
+          if (m & (1 << (l - 1))) v += (unsigned)W.w[l];
+      }
+    }
+  }
+  red[threadIdx.x] = v;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0 && red[0] != 0) atomicAdd(out, (unsigned long long)red[0]);
+}
+
 /* Pull flags: bit l-1 = bounce along l at x, i.e. bit l-1 of mask(x - c_l)
  * or bit lbar-1 of mask(x); bit 14 = solid. */
 __global__ void __launch_bounds__(256)
@@ -1116,6 +1148,19 @@ extern "C" void slbd_get_objidx(slbd_lattice *L, int32_t *objidx) {
 extern "C" void slbd_makemask(slbd_lattice *L) {
   hipLaunchKernelGGL(k_makemask, plane_grid(L), dim3(256), 0, L->stream, L->objidx, L->mask, L->G);
   LAUNCHCHECK();
+}
+extern "C" long slbd_countmask(slbd_lattice *L, int faces, const int *w) {
+  CountW W;
+  for (int l = 0; l < 15; l++) W.w[l] = w[l];
+  /* the reduction result slot serves as the counter */
+  unsigned long long *cnt = (unsigned long long *)L->dout, h = 0;
+  HIPCHECK(hipMemsetAsync(cnt, 0, sizeof h, L->stream));
+  hipLaunchKernelGGL(k_countmask, lin_grid(L->N), dim3(256), 0, L->stream, L->mask, L->N,
+                     faces, W, cnt);
+  LAUNCHCHECK();
+  HIPCHECK(hipMemcpyAsync(&h, cnt, sizeof h, hipMemcpyDeviceToHost, L->stream));
+  HIPCHECK(hipStreamSynchronize(L->stream));
+  return (long)h;
 }
 extern "C" void slbd_put_mask(slbd_lattice *L, const int32_t *mask) {
   copy_in(L->mask, mask, sizeof(int32_t) * L->N, L->stream);

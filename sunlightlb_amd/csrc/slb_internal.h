@@ -37,6 +37,18 @@ static inline void *slb_xrealloc(void *q, size_t n) {
   return p;
 }
 
+/* O(N) host passes split over host threads (slb_par.c): fn(ctx, lo, hi, tid)
+ * runs on contiguous ranges in order of tid; slb_par_for returns how many
+ * ranges it used, so the caller combines per-range slots 0 .. that - 1. */
+#define SLB_PAR_MAX 64
+typedef void (*slb_range_fn)(void *ctx, long lo, long hi, int tid);
+int slb_host_threads(void);
+int slb_par_for(long n, slb_range_fn fn, void *ctx);
+/* zero-filled per-node host arrays, large ones on 2 MiB pages (slb_par.c);
+ * release with slb_big_free */
+void *slb_big_calloc(long n, size_t size);
+void slb_big_free(void *p);
+
 /* Where the newest copy of a mirrored array lives. */
 enum { LOC_SYNC = 0, LOC_HOST = 1, LOC_DEV = 2 };
 

@@ -42,7 +42,7 @@ MPSys newmp(LBSys lb, char *name) {
   mp->lb = lb;
   mp->p = slb_ralloc(N);
   mp->newp = slb_ralloc(N);
-  if (lb->rho != NULL) free(lb->rho);
+  if (lb->rho != NULL) slb_big_free(lb->rho);
   lb->rho = slb_ralloc(N);
   for (long n = 0; n < N; n++) lb->rho[n] = lb->rho0;
   PRIV(lb)->rho_loc = LOC_HOST;

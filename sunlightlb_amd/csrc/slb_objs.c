@@ -189,11 +189,17 @@ static void axis_range(double c, double r, int n, int *lo, int *hi) {
 }
 
 /* Solid-node predicate of insertsph (src/lbobjs.c:212-218): FP64, the three
- * squares summed left to right. */
-static inline int in_sphere(int i, int j, int k, double ox, double oy, double oz, double radsq) {
-  volatile double dx = i - ox, dy = j - oy, dz = k - oz;
-  double rsq = dx * dx + dy * dy + dz * dz;
-  return rsq < radsq;
+ * squares summed left to right, (dx^2 + dy^2) + dz^2 (this file is built
+ * with -ffp-contract=off, so no product is fused into a sum).  A row (i, j)
+ * whose dx^2 + dy^2 already reaches rad^2 holds no node of the sphere:
+ * adding dz^2 >= 0 cannot round the sum below it. */
+static inline double row_part(int i, int j, double ox, double oy) {
+  const double dx = i - ox, dy = j - oy;
+  return dx * dx + dy * dy;
+}
+static inline int in_sphere_row(double part, int k, double oz, double radsq) {
+  const double dz = k - oz;
+  return part + dz * dz < radsq;
 }
 
 /* Rasterise one sphere into obj[] with object p; returns the node count. */
@@ -205,14 +211,17 @@ static int raster_sphere(LBSys lb, LBObj p, double rad, double ox, double oy, do
   axis_range(oy, ra, lb->esj, &jlo, &jhi);
   axis_range(oz, ra, lb->esk, &klo, &khi);
   for (int i = ilo; i <= ihi; i++)
-    for (int j = jlo; j <= jhi; j++)
+    for (int j = jlo; j <= jhi; j++) {
+      const double part = row_part(i, j, ox, oy);
+      if (!(part < radsq)) continue;
       for (int k = klo; k <= khi; k++)
-        if (in_sphere(i, j, k, ox, oy, oz, radsq)) {
+        if (in_sphere_row(part, k, oz, radsq)) {
           long n = nidx(lb, i, j, k);
           if (lb->obj[n] != NULL) *overlap = 1;
           lb->obj[n] = p;
           ns++;
         }
+    }
   return ns;
 }
 
@@ -365,22 +374,54 @@ static void free_list(LBSys lb) {
   lb->obj_head = NULL;
 }
 
+/* flattenobj / invertobj pass: the nodes that end up in the new object p
+ * (the solid ones, or for invert the fluid ones, whose solid neighbours turn
+ * fluid), with per-range node counts and index sums.  The reference sums the
+ * indices as doubles in node order; the integer sums here, converted once,
+ * are the same numbers (every partial sum is an integer below 2^53). */
+typedef struct {
+  LBSys lb;
+  LBObj p;
+  int invert;
+  long ns[SLB_PAR_MAX];
+  long long si[SLB_PAR_MAX], sj[SLB_PAR_MAX], sk[SLB_PAR_MAX];
+} merge_pass;
+
+static void merge_range(void *c, long lo, long hi, int tid) {
+  merge_pass *J = (merge_pass *)c;
+  LBSys lb = J->lb;
+  const long esk = lb->esk, plane = (long)lb->esj * esk;
+  long ns = 0;
+  long long si = 0, sj = 0, sk = 0;
+  for (long n = lo; n < hi; n++) {
+    const int take = J->invert ? (lb->obj[n] == NULL) : (lb->obj[n] != NULL);
+    if (J->invert && !take) { lb->obj[n] = NULL; continue; }
+    if (!take) continue;
+    const long i = n / plane, r = n - i * plane, j = r / esk, k = r - j * esk;
+    si += i; sj += j; sk += k; ns++;
+    lb->obj[n] = J->p;
+  }
+  J->ns[tid] = ns; J->si[tid] = si; J->sj[tid] = sj; J->sk[tid] = sk;
+}
+
+static long merge_nodes(LBSys lb, LBObj p, int invert, LBReal *ox, LBReal *oy, LBReal *oz) {
+  merge_pass J;
+  J.lb = lb; J.p = p; J.invert = invert;
+  const int nr = slb_par_for(lb->nnodes, merge_range, &J);
+  long ns = 0;
+  long long si = 0, sj = 0, sk = 0;
+  for (int t = 0; t < nr; t++) { ns += J.ns[t]; si += J.si[t]; sj += J.sj[t]; sk += J.sk[t]; }
+  *ox = (LBReal)si; *oy = (LBReal)sj; *oz = (LBReal)sk;
+  return ns;
+}
+
 /* All solid nodes become one object at their centroid (src/lbobjs.c:362-392). */
 LBObj flattenobj(LBSys lb) {
   free_list(lb);
   LBObj p = addobj(lb);
   zeroobj(p);
-  int ns = 0;
   LBReal ox = 0.0, oy = 0.0, oz = 0.0;
-  for (int i = 0; i < lb->esi; i++)
-    for (int j = 0; j < lb->esj; j++)
-      for (int k = 0; k < lb->esk; k++) {
-        long n = nidx(lb, i, j, k);
-        if (lb->obj[n] != NULL) {
-          ox += (double)i; oy += (double)j; oz += (double)k; ns++;
-          lb->obj[n] = p;
-        }
-      }
+  int ns = (int)merge_nodes(lb, p, 0, &ox, &oy, &oz);
   if (ns == 0) {
     if (slb_verbosity & OBJECT_DETAILS) printf("No sites set in object\n");
     drop_head(lb, p);
@@ -400,18 +441,8 @@ LBObj invertobj(LBSys lb) {
   free_list(lb);
   LBObj p = addobj(lb);
   zeroobj(p);
-  int ns = 0;
   LBReal ox = 0.0, oy = 0.0, oz = 0.0;
-  for (int i = 0; i < lb->esi; i++)
-    for (int j = 0; j < lb->esj; j++)
-      for (int k = 0; k < lb->esk; k++) {
-        long n = nidx(lb, i, j, k);
-        if (lb->obj[n] != NULL) lb->obj[n] = NULL;
-        else {
-          ox += (double)i; oy += (double)j; oz += (double)k; ns++;
-          lb->obj[n] = p;
-        }
-      }
+  int ns = (int)merge_nodes(lb, p, 1, &ox, &oy, &oz);
   slb_obj_changed(lb);
   if (ns == 0) {
     if (slb_verbosity & OBJECT_DETAILS) printf("No sites set in object\n");
@@ -501,6 +532,26 @@ void wrmap(LBSys lb, char *name) {
   printf("\n");
   free(count);
   printf("Map of object space written to %s\n", name);
+}
+
+typedef struct {
+  LBSys lb;
+  long n[SLB_PAR_MAX];
+} count_pass;
+static void solid_range(void *c, long lo, long hi, int tid) {
+  count_pass *J = (count_pass *)c;
+  long m = 0;
+  for (long n = lo; n < hi; n++) m += J->lb->obj[n] != NULL;
+  J->n[tid] = m;
+}
+/* solid nodes of obj[] */
+static long slb_count_solid(LBSys lb) {
+  count_pass J;
+  J.lb = lb;
+  const int nr = slb_par_for(lb->nnodes, solid_range, &J);
+  long m = 0;
+  for (int t = 0; t < nr; t++) m += J.n[t];
+  return m;
 }
 
 /* ------------------------------------------------------------------ */
@@ -603,8 +654,7 @@ int slb_rsa_pack(LBSys lb, double R, double phi, unsigned long long seed, long m
   g.head = (int *)slb_xmalloc(sizeof(int) * ncell);
   for (int c = 0; c < ncell; c++) g.head[c] = -1;
   long target = (long)ceil(phi * (double)lb->nnodes);
-  long solid = 0;
-  for (long n = 0; n < lb->nnodes; n++) solid += lb->obj[n] != NULL;
+  long solid = slb_count_solid(lb);
   int placed = 0;
   int saved = slb_verbosity;
   slb_verbosity &= ~OBJECT_DETAILS;
@@ -671,12 +721,15 @@ static long raster_map(unsigned char *solid, const int *L, double R, const doubl
   const double radsq = R * R;
   long added = 0;
   for (int i = lo[0]; i <= hi[0]; i++)
-    for (int j = lo[1]; j <= hi[1]; j++)
+    for (int j = lo[1]; j <= hi[1]; j++) {
+      const double part = row_part(i, j, o[0], o[1]);
+      if (!(part < radsq)) continue;
       for (int k = lo[2]; k <= hi[2]; k++)
-        if (in_sphere(i, j, k, o[0], o[1], o[2], radsq)) {
+        if (in_sphere_row(part, k, o[2], radsq)) {
           long n = (long)L[2] * ((long)L[1] * i + j) + k;
           if (!solid[n]) { solid[n] = 1; added++; }
         }
+    }
   return added;
 }
 
@@ -726,12 +779,28 @@ int slb_pack_map(int esi, int esj, int esk, double R, double phi, unsigned long 
   return placed;
 }
 
+typedef struct {
+  LBSys lb;
+  LBObj p;
+  const unsigned char *solid;
+  long ns[SLB_PAR_MAX];
+} map_pass;
+static void map_range(void *c, long lo, long hi, int tid) {
+  map_pass *J = (map_pass *)c;
+  long ns = 0;
+  for (long n = lo; n < hi; n++)
+    if (J->solid[n]) { J->lb->obj[n] = J->p; ns++; }
+  J->ns[tid] = ns;
+}
+
 LBObj slb_insert_map(LBSys lb, const unsigned char *solid) {
   LBObj p = addobj(lb);
   zeroobj(p);
+  map_pass J;
+  J.lb = lb; J.p = p; J.solid = solid;
+  const int nr = slb_par_for(lb->nnodes, map_range, &J);
   long ns = 0;
-  for (long n = 0; n < lb->nnodes; n++)
-    if (solid[n]) { lb->obj[n] = p; ns++; }
+  for (int t = 0; t < nr; t++) ns += J.ns[t];
   if (ns == 0) {
     drop_head(lb, p);
     lb->next_id--;
