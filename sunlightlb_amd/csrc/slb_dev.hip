@@ -375,13 +375,12 @@ __global__ void __launch_bounds__(256)
 k_countmask(const int32_t *__restrict__ mask, long N, int faces, CountW W,
             unsigned long long *__restrict__ out) {
   __shared__ unsigned red[256];
-  const long x = (long)blockIdx.x * 256 + threadIdx.x;
   unsigned v = 0;
-  if (x < N) {
+  for (long x = (long)blockIdx.x * 256 + threadIdx.x; x < N; x += (long)gridDim.x * 256) {
     const int m = mask[x];
     if (m > 0) {
       if (faces) {
-        v = (unsigned)__popc((unsigned)m & 0x3fu);
+        v += (unsigned)__popc((unsigned)m & 0x3fu);
       } else {
         for (int l = 1; l < 15; l++)
           if (m & (1 << (l - 1))) v += (unsigned)W.w[l];
@@ -828,6 +827,162 @@ k_mpstep(const T *__restrict__ a, const T *__restrict__ qin, T *__restrict__ qou
   ((V *)qout)[x] = o;
 }
 
+/* Transition weights of the MP gather for a frozen flow.  Over the steps of
+ * one run the populations, the density, alpha, the adsorption probability
+ * and the object velocities do not change, so the factor that a link l
+ * carries from its source y = x - c_l,
+ *   pr_l(x) = a_l(y) / rho(y) + (w_l / 72) (rho0 / rho(y) - alpha)
+ * (zero when y is solid; the rest link adds alpha), is the same at every
+ * step, and so is the sum over the partially adsorbing links of x of the
+ * factor its own tracer keeps (slot 15).  k_mpweights stores the 16 of
+ * each node; k_mpstep_w then reads them as four vector loads instead of 15
+ * pulled population loads (and the own-node loads of partially adsorbing
+ * links), and gathers only the records.  The one difference in arithmetic:
+ * the kept own tracer is (sum of the link factors) x P rather than the sum
+ * of the products. */
+/* weights in tiles of 1024 nodes: four runs (weights 4c .. 4c+3) of 1024
+ * four-value vectors, so a wave's load of one run is 1 KiB contiguous */
+template <typename I>
+__device__ __forceinline__ I wt_idx(int c, I x) {
+  return (((x >> POP_SHIFT) * I(4) + I(c)) * I(POP_TILE) + (x & I(POP_TILE - 1))) * I(4);
+}
+
+template <typename T, bool MOVING, typename I>
+__global__ void __launch_bounds__(256)
+k_mpweights(const T *__restrict__ a, const T *__restrict__ qin, T *__restrict__ wt,
+            const uint32_t *__restrict__ mpf, const int32_t *__restrict__ objidx,
+            const double *__restrict__ tab, int W, Geom G, MPArgs A) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= G.plane) return;
+  const int i = blockIdx.y, j = t / G.esk, k = t - j * G.esk;
+  const I x = (I)i * (I)G.plane + (I)t;
+  const AxisOffT<I> oi = axis_off_t<I>(i, G.esi, (I)G.plane);
+  const AxisOffT<I> oj = axis_off_t<I>(j, G.esj, (I)G.esk);
+  const AxisOffT<I> ok = axis_off_t<I>(k, G.esk, (I)1);
+  const uint32_t f = mpf[x];
+  const T r0 = T(A.rho0), al = T(A.alpha);
+  T w[16];
+  const T rix = qin[(long)x * W + W - 1];
+  w[0] = (f & 1u) ? a[pidx_t<I>(0, x)] * rix + T(16.0 / 72.0) * (r0 * rix - al) + al : T(0);
+  Links<1, 15>::run([&](auto LL) {
+    constexpr int l = decltype(LL)::value;
+    const I y = x + nb_off<l, -1>(oi, oj, ok);
+    const T ri = qin[(long)y * W + W - 1];
+    const T pr0 = a[pidx_t<I>(l, y)] * ri + T(WL(l) / 72.0) * (r0 * ri - al);
+    w[l] = (f & (1u << l)) ? pr0 : T(0);
+  });
+  T R = T(0);
+  if (f >> 17) {
+    const T keep = T(1.0 - A.pads);
+    Links<1, 15>::run([&](auto LL) {
+      constexpr int l = decltype(LL)::value;
+      if (f & (1u << (16 + l))) {
+        T pr = a[pidx_t<I>(OPP(l), x)] * rix + T(WL(l) / 72.0) * (r0 * rix - al);
+        if (MOVING) {
+          const I s = x + nb_off<l, -1>(oi, oj, ok);
+          const int si = (i - CX(l) + G.esi) % G.esi;
+          const int sj = (j - CY(l) + G.esj) % G.esj;
+          const int sk = (k - CZ(l) + G.esk) % G.esk;
+          double Ux, Uy, Uz;
+          obj_velocity(tab, objidx[s], si, sj, sk, A.rho0, Ux, Uy, Uz);
+          const double cu = (double)WL(l) * (CX(l) * Ux + CY(l) * Uy + CZ(l) * Uz);
+          pr += (T)(cu * (double)rix);
+        }
+        R += pr * keep;
+      }
+    });
+  }
+  w[15] = R;
+  typedef T V4 __attribute__((ext_vector_type(4)));
+#pragma unroll
+  for (int c = 0; c < 4; c++)
+    *(V4 *)(wt + wt_idx<I>(c, x)) = (V4){w[4 * c], w[4 * c + 1], w[4 * c + 2], w[4 * c + 3]};
+}
+
+/* k_mpstep with the gather weights precomputed by k_mpweights: the same
+ * sums over the same links in the same order. */
+template <typename T, int NMOM, bool MOVING, typename I>
+__global__ void __launch_bounds__(256)
+k_mpstep_w(const T *__restrict__ a, const T *__restrict__ wt, const T *__restrict__ qin,
+           T *__restrict__ qout, const uint32_t *__restrict__ mpf,
+           const int32_t *__restrict__ mask, const int32_t *__restrict__ objidx,
+           const double *__restrict__ tab, Geom G, MPArgs A) {
+  constexpr int W = NMOM > 1 ? 4 : 2;
+  typedef T V __attribute__((ext_vector_type(W)));
+  typedef T V4 __attribute__((ext_vector_type(4)));
+  const V *__restrict__ Q = (const V *)qin;
+  Geom Gb = G;
+  Gb.xcd_map = G.mp_xcd;
+  int blk_x, blk_y;
+  step_block(Gb, blk_x, blk_y);
+  const int t = blk_x * blockDim.x + threadIdx.x;
+  if (t >= G.plane) return;
+  const int i = blk_y, j = t / G.esk, k = t - j * G.esk;
+  const I x = (I)i * (I)G.plane + (I)t;
+  const AxisOffT<I> oi = axis_off_t<I>(i, G.esi, (I)G.plane);
+  const AxisOffT<I> oj = axis_off_t<I>(j, G.esj, (I)G.esk);
+  const AxisOffT<I> ok = axis_off_t<I>(k, G.esk, (I)1);
+  const uint32_t f = mpf[x];
+  const T r0 = T(A.rho0), al = T(A.alpha);
+  const V qx = Q[x];
+  const T rix = qx[W - 1];
+  T wv[16];
+  {
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+      const V4 v = *(const V4 *)(wt + wt_idx<I>(c, x));
+      wv[4 * c] = v[0]; wv[4 * c + 1] = v[1]; wv[4 * c + 2] = v[2]; wv[4 * c + 3] = v[3];
+    }
+  }
+  V qv[15];
+  Links<1, 15>::run([&](auto LL) {
+    constexpr int l = decltype(LL)::value;
+    qv[l] = Q[x + nb_off<l, -1>(oi, oj, ok)];
+  });
+  T np = T(0), n1 = T(0), n2 = T(0);
+  if (f & 1u) {
+    const T pr = wv[0];
+    np += pr * qx[0];
+    if (NMOM > 1) { n1 += pr * qx[1]; n2 += pr * qx[2]; }
+  }
+  Links<1, 15>::run([&](auto LL) {
+    constexpr int l = decltype(LL)::value;
+    const V qy = qv[l];
+    const T P = qy[0];
+    const T pr = wv[l];
+    np += pr * P;
+    if (NMOM > 1) {
+      const int dd = (A.axis == 0) ? CX(l) : (A.axis == 1) ? CY(l) : CZ(l);
+      const T q1 = qy[1];
+      n1 += pr * (q1 + T(dd) * P);
+      n2 += pr * (qy[2] + T(2 * dd) * q1 + T(dd * dd) * P);
+    }
+  });
+  if (f >> 17) {
+    const T R = wv[15];
+    np += R * qx[0];
+    if (NMOM > 1) { n1 += R * qx[1]; n2 += R * qx[2]; }
+  }
+  V o;
+  o[W - 1] = rix;
+  o[0] = T(0);
+  if (NMOM > 1) { o[1] = T(0); o[2] = T(0); }
+  if (f & 1u) {
+    o[0] = np;
+    if (NMOM > 1) { o[1] = n1; o[2] = n2; }
+  } else {
+    const int m = mask[x];
+    T frac = T(0);
+    if (m & 0x4000) frac = T(1);
+    else if (m != 0 && A.pads > 0.0) frac = T(A.pads);
+    if (frac != T(0)) {
+      o[0] = qx[0] + frac * np;
+      if (NMOM > 1) { o[1] = qx[1] + frac * n1; o[2] = qx[2] + frac * n2; }
+    }
+  }
+  ((V *)qout)[x] = o;
+}
+
 /* record component access: q[x * W + c] */
 template <typename T>
 __global__ void k_rec_put(T *__restrict__ q, int W, int c, const double *__restrict__ v, long N) {
@@ -928,6 +1083,7 @@ struct slbd_mp {
   int cur, nmom, prec, W;
   long N;
   long rho_version;  /* lattice density version the 1/rho slots hold */
+  void *wt;        /* gather weights of the current run (16 per node, tiled), or NULL */
 };
 
 static size_t tsize(int prec) { return prec == 32 ? sizeof(float) : sizeof(double); }
@@ -1155,8 +1311,10 @@ extern "C" long slbd_countmask(slbd_lattice *L, int faces, const int *w) {
   /* the reduction result slot serves as the counter */
   unsigned long long *cnt = (unsigned long long *)L->dout, h = 0;
   HIPCHECK(hipMemsetAsync(cnt, 0, sizeof h, L->stream));
-  hipLaunchKernelGGL(k_countmask, lin_grid(L->N), dim3(256), 0, L->stream, L->mask, L->N,
-                     faces, W, cnt);
+  /* a grid-stride pass over at most 2048 blocks: few atomics on the counter */
+  const long nb = (L->N + 255) / 256;
+  hipLaunchKernelGGL(k_countmask, dim3((unsigned)(nb < 2048 ? nb : 2048)), dim3(256), 0,
+                     L->stream, L->mask, L->N, faces, W, cnt);
   LAUNCHCHECK();
   HIPCHECK(hipMemcpyAsync(&h, cnt, sizeof h, hipMemcpyDeviceToHost, L->stream));
   HIPCHECK(hipStreamSynchronize(L->stream));
@@ -1925,6 +2083,7 @@ extern "C" void slbd_mp_destroy(slbd_mp *M) {
   HIPCHECK(hipDeviceSynchronize());
   for (int b = 0; b < 2; b++)
     if (M->q[b]) HIPCHECK(hipFree(M->q[b]));
+  if (M->wt) HIPCHECK(hipFree(M->wt));
   free(M);
 }
 
@@ -2026,6 +2185,46 @@ extern "C" void slbd_mp_zero(slbd_lattice *L, slbd_mp *M, int field) {
   LAUNCHCHECK();
 }
 
+/* MP steps of one run with precomputed gather weights: runs of at least
+ * this many steps (SLB_MP_WEIGHTS=0 turns the weights off) */
+static int mp_weights_min(void) {
+  static int v = -1;
+  if (v < 0) {
+    const char *e = getenv("SLB_MP_WEIGHTS");
+    v = (e && atoi(e) == 0) ? 0 : 4;
+  }
+  return v;
+}
+
+template <typename T, int NMOM, bool MOVING>
+static void launch_mp_w(slbd_lattice *L, slbd_mp *M, const MPArgs &A, int nsteps) {
+  if (!M->wt) M->wt = dalloc((size_t)pop_nodes(L->N) * 16 * sizeof(T));
+  if (L->idx32)
+    hipLaunchKernelGGL((k_mpweights<T, MOVING, int>), plane_grid(L), dim3(256), 0, L->stream,
+                       (const T *)L->pops[L->cur], (const T *)M->q[M->cur], (T *)M->wt, L->mpf,
+                       L->objidx, L->tab, M->W, L->G, A);
+  else
+    hipLaunchKernelGGL((k_mpweights<T, MOVING, long>), plane_grid(L), dim3(256), 0, L->stream,
+                       (const T *)L->pops[L->cur], (const T *)M->q[M->cur], (T *)M->wt, L->mpf,
+                       L->objidx, L->tab, M->W, L->G, A);
+  LAUNCHCHECK();
+  for (int s = 0; s < nsteps; s++) {
+    const int c = M->cur;
+    if (!MOVING && L->idx32)
+      hipLaunchKernelGGL((k_mpstep_w<T, NMOM, MOVING, int>), plane_grid(L), dim3(256), 0,
+                         L->stream, (const T *)L->pops[L->cur], (const T *)M->wt,
+                         (const T *)M->q[c], (T *)M->q[c ^ 1], L->mpf, L->mask, L->objidx,
+                         L->tab, L->G, A);
+    else
+      hipLaunchKernelGGL((k_mpstep_w<T, NMOM, MOVING, long>), plane_grid(L), dim3(256), 0,
+                         L->stream, (const T *)L->pops[L->cur], (const T *)M->wt,
+                         (const T *)M->q[c], (T *)M->q[c ^ 1], L->mpf, L->mask, L->objidx,
+                         L->tab, L->G, A);
+    LAUNCHCHECK();
+    M->cur ^= 1;
+  }
+}
+
 template <typename T, int NMOM, bool MOVING>
 static void launch_mp_t(slbd_lattice *L, slbd_mp *M, const MPArgs &A) {
   const int c = M->cur;
@@ -2043,11 +2242,24 @@ static void launch_mp_t(slbd_lattice *L, slbd_mp *M, const MPArgs &A) {
 }
 
 template <typename T>
-static void launch_mp(slbd_lattice *L, slbd_mp *M, const MPArgs &A, bool moving) {
-  if (M->nmom > 1) {
-    if (moving) launch_mp_t<T, 3, true>(L, M, A); else launch_mp_t<T, 3, false>(L, M, A);
-  } else {
-    if (moving) launch_mp_t<T, 1, true>(L, M, A); else launch_mp_t<T, 1, false>(L, M, A);
+static void launch_mp(slbd_lattice *L, slbd_mp *M, const MPArgs &A, bool moving, int nsteps) {
+  const int wmin = mp_weights_min();
+  if (wmin > 0 && nsteps >= wmin) {
+    if (M->nmom > 1) {
+      if (moving) launch_mp_w<T, 3, true>(L, M, A, nsteps);
+      else launch_mp_w<T, 3, false>(L, M, A, nsteps);
+    } else {
+      if (moving) launch_mp_w<T, 1, true>(L, M, A, nsteps);
+      else launch_mp_w<T, 1, false>(L, M, A, nsteps);
+    }
+    return;
+  }
+  for (int s = 0; s < nsteps; s++) {
+    if (M->nmom > 1) {
+      if (moving) launch_mp_t<T, 3, true>(L, M, A); else launch_mp_t<T, 3, false>(L, M, A);
+    } else {
+      if (moving) launch_mp_t<T, 1, true>(L, M, A); else launch_mp_t<T, 1, false>(L, M, A);
+    }
   }
 }
 
@@ -2061,10 +2273,8 @@ extern "C" void slbd_mp_step(slbd_lattice *L, slbd_mp *M, double rho0, double al
   if (moving) put_tab(L, nobj, objtab);
   MPArgs A;
   A.rho0 = rho0; A.alpha = alpha; A.pads = pads; A.axis = axis;
-  for (int s = 0; s < nsteps; s++) {
-    if (L->prec == 32) launch_mp<float>(L, M, A, moving != 0);
-    else launch_mp<double>(L, M, A, moving != 0);
-  }
+  if (L->prec == 32) launch_mp<float>(L, M, A, moving != 0, nsteps);
+  else launch_mp<double>(L, M, A, moving != 0, nsteps);
 }
 
 extern "C" double slbd_time_mp(slbd_lattice *L, slbd_mp *M, double rho0, double alpha,

@@ -133,8 +133,12 @@ def test_moving_walls_match_reference():
     assert rel_l2(vm, g["meas_vmean"]) < TOL64
 
 
+@pytest.mark.parametrize("stepping", ["mpstep", "run"])
 @pytest.mark.parametrize("variant", ["reflect", "adsorb", "moving"])
-def test_mp_matches_reference(variant):
+def test_mp_matches_reference(variant, stepping):
+    """25 MP steps against the reference library's fields: one mpstep call
+    per step, or one slb_mp_run call (whose steps share gather weights
+    computed once from the frozen flow)."""
     g = load_golden("mp.npz")
     dims = tuple(g["meta"]["dims"])
     lat = lattice(dims)
@@ -158,8 +162,11 @@ def test_mp_matches_reference(variant):
         L.adall(mp)
     assert rel_l2(lat.pops(), g[f"{variant}_a"]) < TOL64
     assert rel_l2(lat.field("rho"), g[f"{variant}_rho"]) < 1e-15
-    for _ in range(25):
-        L.mpstep(mp)
+    if stepping == "mpstep":
+        for _ in range(25):
+            L.mpstep(mp)
+    else:
+        L.slb_mp_run(mp, 25)
     L.slb_mp_sync_host(mp)
     import sunlightlb_amd as s
     p = s.view(mp.contents.p, lat.N).copy()
