@@ -955,7 +955,10 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / PEAK_HBM_GBS, 4),
             "traffic": traffic_bytes(n, args.precision),
-            "kernel": "k_step_lds (fused pull step, k pulls staged in LDS)",
+            # the library stages the k pulls in LDS for rows of 64-512 nodes
+            "kernel": ("k_step_lds (fused pull step, k pulls staged in LDS)"
+                       if args.size in (64, 128, 256, 512) and os.environ.get("SLB_STEP_LDS", "1") != "0"
+                       else "k_step (fused pull step)"),
             "algorithmic_bytes_per_node": bytes_per_lup,
             "device_copy_GBs": round(copy_gbs, 1),
         },

@@ -165,7 +165,10 @@ k_step(const T *__restrict__ src, T *__restrict__ dst,
  * of one shifted by an element (two 128-byte lines per wave and population
  * instead of three).  Links with cz = 0 and bounced links (a node's own
  * opposite population) load from global memory as in k_step.  Same values,
- * same arithmetic: bit-identical to k_step. */
+ * same arithmetic: bit-identical to k_step.  (512-node segments of longer
+ * rows, with the two values beyond a segment's ends staged by its end
+ * threads, measured no faster than k_step at 1024^3 and slower at rows of
+ * 2048: 13.7 against 12.8 ms at 512 x 512 x 2048.) */
 template <typename T, bool MOVING, bool FORCE, int BF, typename I>
 __global__ void __launch_bounds__(512)
 k_step_lds(const T *__restrict__ src, T *__restrict__ dst,
@@ -176,8 +179,10 @@ k_step_lds(const T *__restrict__ src, T *__restrict__ dst,
            double *__restrict__ fx, double *__restrict__ fy, double *__restrict__ fz,
            Geom G, PhysArgs P) {
   __shared__ T row[10][512];
-  const int t = blockIdx.x * 512 + threadIdx.x;     /* plane % 512 == 0: all in range */
-  const int i = blockIdx.y;
+  int blk_x, blk_y;
+  step_block(G, blk_x, blk_y);
+  const int t = blk_x * 512 + threadIdx.x;          /* plane % 512 == 0: all in range */
+  const int i = blk_y;
   const int kmask = G.esk - 1;
   const int j = t / G.esk;
   const int k = t & kmask;
@@ -1115,8 +1120,12 @@ static int step_order(const Geom &G) {
     if (m == 0 || m == 1) return (m == 1 && nb % 8 == 0) ? 1 : 0;
     return fits(m) ? m : 0;
   }
-  /* measured on the tiled population layout: the plain order is fastest */
-  return 0;
+  /* measured on the tiled population layout: the plain order is fastest at
+   * 256^3; from planes of 2^19 nodes on, the pulls from planes i - 1 and
+   * i + 1 lie 30 MB and more apart, and each XCD walking a contiguous range
+   * of blocks keeps the span of its loads compact: 27.0 -> 24.0-25.2 ms per
+   * FP32 step at 1024^3, 14.9 -> 12.8 ms at 512 x 512 x 2048 */
+  return (G.plane >= (1 << 19) && nb % 8 == 0) ? 1 : 0;
 }
 
 extern "C" int slbd_device_count(void) {

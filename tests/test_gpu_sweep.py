@@ -107,10 +107,11 @@ def test_sweep_fp64_matches_oracle(oracle, monkeypatch):
 
 
 @pytest.mark.parametrize("prec", [64, 32])
-@pytest.mark.parametrize("dims", [(12, 16, 64), (9, 8, 128), (6, 6, 256), (1, 1, 512), (3, 2, 256)])
+@pytest.mark.parametrize("dims", [(12, 16, 64), (9, 8, 128), (6, 6, 256), (1, 1, 512), (3, 2, 256),
+                                  (2, 1024, 512)])
 def test_lds_staged_step_bit_identical(prec, dims, monkeypatch):
-    """k_step_lds (the k pulls staged in LDS; rows of 64-512 nodes) against
-    k_step: populations after odd and even step counts and the node forces
+    """k_step_lds (the k pulls staged in LDS; rows of 64-512 nodes; planes of
+    2^19 nodes and more in the XCD-contiguous block order) against k_step: populations after odd and even step counts and the node forces
     of measure(), moving walls, Navier-Stokes, per-node body force."""
     import sunlightlb_amd as s
 
