@@ -908,6 +908,7 @@ k_mpweights(const T *__restrict__ a, const T *__restrict__ qin, T *__restrict__ 
  * sums over the same links in the same order. */
 template <typename T, int NMOM, bool MOVING, typename I>
 __global__ void __launch_bounds__(256)
+__attribute__((amdgpu_waves_per_eu(1, sizeof(T) == 8 ? 4 : 8)))
 k_mpstep_w(const T *__restrict__ a, const T *__restrict__ wt, const T *__restrict__ qin,
            T *__restrict__ qout, const uint32_t *__restrict__ mpf,
            const int32_t *__restrict__ mask, const int32_t *__restrict__ objidx,
