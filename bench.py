@@ -50,8 +50,9 @@ def build_pack(L, n, R, phi, seed, force):
     s.tau = 0.75
     phi_out = C.c_double(0.0)
     t0 = time.perf_counter()
-    nsph = L.slb_rsa_pack(h, R, phi, seed, 50_000_000, None, 0, C.byref(phi_out))
-    L.flattenobj(h)
+    # random sequential addition and flattenobj in one call, the sphere
+    # images counted and rasterised on the GPU
+    nsph = L.slb_rsa_pack_flat(h, R, phi, seed, 50_000_000, None, 0, C.byref(phi_out))
     L.zerobf(h)
     L.addbf(h, force, 0.0, 0.0)
     L.syncobjs(h)

@@ -98,6 +98,15 @@ int slb_insert_spheres(LBSys lb, int n, const double *rxyz);
 int slb_rsa_pack(LBSys lb, double R, double phi, unsigned long long seed,
                  long maxtrials, double *centres, int maxcentres, double *phi_out);
 
+/* slb_rsa_pack followed by flattenobj() in one call, with the node counts
+ * and the rasterisation of the sphere images done on the GPU (the same FP64
+ * predicate): the same spheres, centres, solid fraction, node map and single
+ * object (id 0 at the centroid of every solid node, objects already present
+ * included).  Entries of centres beyond the returned count are
+ * unspecified. */
+int slb_rsa_pack_flat(LBSys lb, double R, double phi, unsigned long long seed,
+                      long maxtrials, double *centres, int maxcentres, double *phi_out);
+
 /* The solid map (solid[n] = 1, n = esk*(esj*i+j)+k) of a periodic sphere
  * pack of radius R to lattice solid fraction phi, without a lattice:
  * overlap = 0 is the random sequential addition of slb_rsa_pack (same

@@ -157,6 +157,7 @@ SIGNATURES = {
     "slb_sum_pops": (None, [LBSys, RealP]),
     "slb_insert_spheres": (I, [LBSys, I, RealP]),
     "slb_rsa_pack": (I, [LBSys, D, D, C.c_ulonglong, C.c_long, RealP, I, RealP]),
+    "slb_rsa_pack_flat": (I, [LBSys, D, D, C.c_ulonglong, C.c_long, RealP, I, RealP]),
     "slb_pack_map": (I, [I, I, I, D, D, C.c_ulonglong, I, C.c_long, VP, RealP]),
     "slb_insert_map": (LBObj, [LBSys, VP]),
     "slb_mp_set_adsorption": (None, [MPSys, D]), "slb_mp_track_moments": (None, [MPSys, I]),

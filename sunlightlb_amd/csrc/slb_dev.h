@@ -83,6 +83,14 @@ slbd_rsa *slbd_rsa_create(const double L[3], const double cs[3], const int nc[3]
 void slbd_rsa_update(slbd_rsa *R, const int *head, const int *next, const double *xyz, int n);
 void slbd_rsa_filter(slbd_rsa *R, uint64_t seed, long t0, int m, double d2min, uint8_t *pass);
 void slbd_rsa_destroy(slbd_rsa *R);
+/* Sphere images (n rows of centre {ox, oy, oz} and clipped bounding box
+ * {ilo, jlo, klo, ihi, jhi, khi}) against insertsph's FP64 predicate with
+ * r^2 = radsq: node count of each image, or the byte map of all of them
+ * (N bytes, copied to the host). */
+void slbd_sph_count(slbd_lattice *L, const double *o, const int *box, int n, double radsq,
+                    int *counts);
+void slbd_sph_map(slbd_lattice *L, const double *o, const int *box, int n, double radsq,
+                  uint8_t *map);
 
 /* Launch plan of the two-step sweep over esi planes in chunks of P (see
  * sweep2 in slb_dev.hip): rows of {phase, i0, np, sring, dring}, phase 1 =

@@ -1082,6 +1082,8 @@ struct slbd_lattice {
   hipEvent_t ev0, ev1;
   int sweep_nt_mid;         /* two-step sweep: non-temporal window stores */
   int step_lds;             /* step with the k pulls staged in LDS (k_step_lds) */
+  void *sph_buf;            /* sphere images of a pack being placed (grow-only) */
+  size_t sph_cap;
 };
 
 struct slbd_mp {
@@ -1218,7 +1220,7 @@ extern "C" void slbd_destroy(slbd_lattice *L) {
   HIPCHECK(hipStreamSynchronize(L->stream));
   void *ps[] = {L->pops[0], L->pops[1], L->flags, L->objidx, L->mask, L->ux, L->uy, L->uz,
                 L->fx, L->fy, L->fz, L->bx, L->by, L->bz, L->part, L->dout, L->psi,
-                L->tab, L->acc, L->uprev};
+                L->tab, L->acc, L->uprev, L->sph_buf};
   for (size_t q = 0; q < sizeof ps / sizeof ps[0]; q++)
     if (ps[q]) HIPCHECK(hipFree(ps[q]));
   free_mp(L);
@@ -3094,6 +3096,98 @@ extern "C" void slbd_rsa_filter(slbd_rsa *R, uint64_t seed, long t0, int m, doub
   LAUNCHCHECK();
   HIPCHECK(hipMemcpyAsync(pass, R->pass, m, hipMemcpyDeviceToHost, R->stream));
   HIPCHECK(hipStreamSynchronize(R->stream));
+}
+
+/* Sphere images of a pack on the device: for each image, the nodes of its
+ * bounding box (clipped to the lattice by the caller, one node of padding)
+ * inside the sphere by insertsph's predicate (src/lbobjs.c:212-218),
+ * (i - ox)^2 + (j - oy)^2 + (k - oz)^2 < r^2 in FP64 with every product and
+ * sum rounded on its own (contraction into fused multiply-adds is off in
+ * the kernel, as in the host code built with -ffp-contract=off) and the
+ * squares added left to right.  mode 0 counts each image's nodes, mode 1
+ * marks them in a byte map. */
+struct SphImg {
+  double o[3];
+  int lo[3], hi[3];
+};
+
+__global__ void __launch_bounds__(256)
+k_sph_images(const SphImg *__restrict__ img, double radsq, int esj, int esk, int mode,
+             int *__restrict__ counts, uint8_t *__restrict__ map) {
+#pragma clang fp contract(off)
+  __shared__ int red[256];
+  const SphImg S = img[blockIdx.x];
+  const int ni = S.hi[0] - S.lo[0] + 1, nj = S.hi[1] - S.lo[1] + 1, nk = S.hi[2] - S.lo[2] + 1;
+  const long tot = (ni > 0 && nj > 0 && nk > 0) ? (long)ni * nj * nk : 0;
+  int c = 0;
+  for (long q = threadIdx.x; q < tot; q += blockDim.x) {
+    const int a = (int)(q / ((long)nj * nk));
+    const int r = (int)(q - (long)a * nj * nk);
+    const int b = r / nk, e = r - b * nk;
+    const int i = S.lo[0] + a, j = S.lo[1] + b, k = S.lo[2] + e;
+    const double dx = (double)i - S.o[0], dy = (double)j - S.o[1], dz = (double)k - S.o[2];
+    const double part = dx * dx + dy * dy;
+    if (part + dz * dz < radsq) {
+      if (mode) map[((long)i * esj + j) * esk + k] = 1;
+      else c++;
+    }
+  }
+  if (mode) return;
+  red[threadIdx.x] = c;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) counts[blockIdx.x] = red[0];
+}
+
+/* images: n rows of {ox, oy, oz} (o) and {ilo, jlo, klo, ihi, jhi, khi} (box),
+ * staged in a grow-only buffer of the lattice (images, then n counts) */
+static SphImg *sph_upload(slbd_lattice *L, const double *o, const int *box, int n) {
+  const size_t need = (sizeof(SphImg) + sizeof(int)) * (size_t)n;
+  if (need > L->sph_cap) {
+    if (L->sph_buf) HIPCHECK(hipFree(L->sph_buf));
+    L->sph_cap = need + need / 2;
+    L->sph_buf = dalloc(L->sph_cap);
+  }
+  SphImg *h = (SphImg *)malloc(sizeof(SphImg) * (size_t)n);
+  if (!h) { fprintf(stderr, "sunlightlb: out of host memory\n"); exit(EXIT_FAILURE); }
+  for (int q = 0; q < n; q++)
+    for (int d = 0; d < 3; d++) {
+      h[q].o[d] = o[3 * q + d];
+      h[q].lo[d] = box[6 * q + d];
+      h[q].hi[d] = box[6 * q + 3 + d];
+    }
+  SphImg *dv = (SphImg *)L->sph_buf;
+  copy_in(dv, h, sizeof(SphImg) * (size_t)n, L->stream);
+  free(h);
+  return dv;
+}
+
+extern "C" void slbd_sph_count(slbd_lattice *L, const double *o, const int *box, int n,
+                               double radsq, int *counts) {
+  if (n <= 0) return;
+  SphImg *dv = sph_upload(L, o, box, n);
+  int *dc = (int *)(dv + n);
+  hipLaunchKernelGGL(k_sph_images, dim3((unsigned)n), dim3(256), 0, L->stream, dv, radsq,
+                     L->esj, L->esk, 0, dc, (uint8_t *)nullptr);
+  LAUNCHCHECK();
+  copy_out(counts, dc, sizeof(int) * (size_t)n, L->stream);
+}
+
+extern "C" void slbd_sph_map(slbd_lattice *L, const double *o, const int *box, int n,
+                             double radsq, uint8_t *map) {
+  uint8_t *dm = (uint8_t *)dalloc((size_t)L->N);
+  HIPCHECK(hipMemsetAsync(dm, 0, (size_t)L->N, L->stream));
+  if (n > 0) {
+    SphImg *dv = sph_upload(L, o, box, n);
+    hipLaunchKernelGGL(k_sph_images, dim3((unsigned)n), dim3(256), 0, L->stream, dv, radsq,
+                       L->esj, L->esk, 1, (int *)nullptr, dm);
+    LAUNCHCHECK();
+  }
+  copy_out(map, dm, (size_t)L->N, L->stream);
+  HIPCHECK(hipFree(dm));
 }
 
 extern "C" void slbd_rsa_destroy(slbd_rsa *R) {

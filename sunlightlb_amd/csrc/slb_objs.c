@@ -383,6 +383,7 @@ typedef struct {
   LBSys lb;
   LBObj p;
   int invert;
+  const unsigned char *map;   /* extra solid nodes (flatten only), or NULL */
   long ns[SLB_PAR_MAX];
   long long si[SLB_PAR_MAX], sj[SLB_PAR_MAX], sk[SLB_PAR_MAX];
 } merge_pass;
@@ -394,7 +395,8 @@ static void merge_range(void *c, long lo, long hi, int tid) {
   long ns = 0;
   long long si = 0, sj = 0, sk = 0;
   for (long n = lo; n < hi; n++) {
-    const int take = J->invert ? (lb->obj[n] == NULL) : (lb->obj[n] != NULL);
+    const int take = J->invert ? (lb->obj[n] == NULL)
+                               : (lb->obj[n] != NULL || (J->map && J->map[n]));
     if (J->invert && !take) { lb->obj[n] = NULL; continue; }
     if (!take) continue;
     const long i = n / plane, r = n - i * plane, j = r / esk, k = r - j * esk;
@@ -404,9 +406,10 @@ static void merge_range(void *c, long lo, long hi, int tid) {
   J->ns[tid] = ns; J->si[tid] = si; J->sj[tid] = sj; J->sk[tid] = sk;
 }
 
-static long merge_nodes(LBSys lb, LBObj p, int invert, LBReal *ox, LBReal *oy, LBReal *oz) {
+static long merge_nodes(LBSys lb, LBObj p, int invert, const unsigned char *map, LBReal *ox,
+                        LBReal *oy, LBReal *oz) {
   merge_pass J;
-  J.lb = lb; J.p = p; J.invert = invert;
+  J.lb = lb; J.p = p; J.invert = invert; J.map = map;
   const int nr = slb_par_for(lb->nnodes, merge_range, &J);
   long ns = 0;
   long long si = 0, sj = 0, sk = 0;
@@ -421,7 +424,7 @@ LBObj flattenobj(LBSys lb) {
   LBObj p = addobj(lb);
   zeroobj(p);
   LBReal ox = 0.0, oy = 0.0, oz = 0.0;
-  int ns = (int)merge_nodes(lb, p, 0, &ox, &oy, &oz);
+  int ns = (int)merge_nodes(lb, p, 0, NULL, &ox, &oy, &oz);
   if (ns == 0) {
     if (slb_verbosity & OBJECT_DETAILS) printf("No sites set in object\n");
     drop_head(lb, p);
@@ -442,7 +445,7 @@ LBObj invertobj(LBSys lb) {
   LBObj p = addobj(lb);
   zeroobj(p);
   LBReal ox = 0.0, oy = 0.0, oz = 0.0;
-  int ns = (int)merge_nodes(lb, p, 1, &ox, &oy, &oz);
+  int ns = (int)merge_nodes(lb, p, 1, NULL, &ox, &oy, &oz);
   slb_obj_changed(lb);
   if (ns == 0) {
     if (slb_verbosity & OBJECT_DETAILS) printf("No sites set in object\n");
@@ -707,6 +710,137 @@ int slb_rsa_pack(LBSys lb, double R, double phi, unsigned long long seed, long m
   slb_obj_changed(lb);
   if (phi_out) *phi_out = (double)solid / (double)lb->nnodes;
   free(g.head); free(g.next); free(g.xyz);
+  return placed;
+}
+
+/* slb_rsa_pack followed by flattenobj, with the sphere images counted and
+ * rasterised on the GPU.  The acceptance of candidates is the same
+ * sequential test in trial order; the spheres accepted from one batch of
+ * trials have their images' node counts taken on the device together, and
+ * the pack ends after the first sphere whose images bring the solid count to
+ * the target, as slb_rsa_pack ends.  The images of the placed spheres are
+ * then marked in a byte map on the device, and every solid node (those of
+ * objects already present and the map's) becomes the one object of
+ * flattenobj.  Same centres, solid fraction, object and node map. */
+typedef struct {
+  double *o;
+  int *box, *sph;
+  int n, cap;
+} img_list;
+
+static void img_push(img_list *L, const double *o, const int *box, int sph) {
+  if (L->n == L->cap) {
+    L->cap = L->cap ? 2 * L->cap : 1024;
+    L->o = (double *)slb_xrealloc(L->o, sizeof(double) * 3 * L->cap);
+    L->box = (int *)slb_xrealloc(L->box, sizeof(int) * 6 * L->cap);
+    L->sph = (int *)slb_xrealloc(L->sph, sizeof(int) * L->cap);
+  }
+  memcpy(L->o + 3 * L->n, o, sizeof(double) * 3);
+  memcpy(L->box + 6 * L->n, box, sizeof(int) * 6);
+  L->sph[L->n] = sph;
+  L->n++;
+}
+
+int slb_rsa_pack_flat(LBSys lb, double R, double phi, unsigned long long seed, long maxtrials,
+                      double *centres, int maxcentres, double *phi_out) {
+  slb_lbpriv *S = PRIV(lb);
+  rsa_grid g;
+  memset(&g, 0, sizeof g);
+  g.L[0] = lb->esi; g.L[1] = lb->esj; g.L[2] = lb->esk;
+  int ncell = 1;
+  for (int d = 0; d < 3; d++) {
+    g.nc[d] = (int)floor(g.L[d] / (2.0 * R));
+    if (g.nc[d] < 1) g.nc[d] = 1;
+    if (g.nc[d] < 3) g.nc[d] = 1;
+    g.cs[d] = g.L[d] / g.nc[d];
+    ncell *= g.nc[d];
+  }
+  g.head = (int *)slb_xmalloc(sizeof(int) * ncell);
+  for (int c = 0; c < ncell; c++) g.head[c] = -1;
+  const long target = (long)ceil(phi * (double)lb->nnodes);
+  long solid = slb_count_solid(lb);
+  const double radsq = R * R, ra = fabs(R);
+  const int dims[3] = {lb->esi, lb->esj, lb->esk};
+  img_list imgs = {NULL, NULL, NULL, 0, 0};
+  int *counts = NULL, ccap = 0;
+  int placed = 0, done = 0;
+  slbd_rsa *D = slbd_rsa_create(g.L, g.cs, g.nc);
+  int m = 1024, uploaded = -1;
+  uint8_t *pass = (uint8_t *)slb_xmalloc(1 << 20);
+  long t = 0;
+  while (t < maxtrials && solid < target && !done) {
+    const int mm = (maxtrials - t < m) ? (int)(maxtrials - t) : m;
+    if (uploaded != g.n) { slbd_rsa_update(D, g.head, g.next, g.xyz, g.n); uploaded = g.n; }
+    slbd_rsa_filter(D, (uint64_t)seed, t, mm, 4.0 * R * R, pass);
+    const int first_img = imgs.n;
+    int npass = 0;
+    for (int q = 0; q < mm; q++) {
+      if (!pass[q]) continue;
+      npass++;
+      double c[3];
+      rsa_centre((uint64_t)seed, t + q, g.L, c);
+      if (rsa_overlaps(&g, c, 4.0 * R * R)) continue;
+      rsa_add(&g, c);
+      for (int a = -1; a <= 1; a++)
+        for (int b = -1; b <= 1; b++)
+          for (int e = -1; e <= 1; e++) {
+            double o[3] = {c[0] + a * g.L[0], c[1] + b * g.L[1], c[2] + e * g.L[2]};
+            if (o[0] + R < 0.0 || o[0] - R > g.L[0] || o[1] + R < 0.0 || o[1] - R > g.L[1] ||
+                o[2] + R < 0.0 || o[2] - R > g.L[2])
+              continue;
+            int box[6];
+            for (int d = 0; d < 3; d++) axis_range(o[d], ra, dims[d], &box[d], &box[3 + d]);
+            img_push(&imgs, o, box, placed);
+          }
+      if (centres && placed < maxcentres) memcpy(centres + 3 * placed, c, sizeof(double) * 3);
+      placed++;
+    }
+    t += mm;
+    /* node counts of this batch's images, then the stopping sphere */
+    const int nb = imgs.n - first_img;
+    if (nb > ccap) { ccap = nb; counts = (int *)slb_xrealloc(counts, sizeof(int) * ccap); }
+    slbd_sph_count(S->dev, imgs.o + 3 * first_img, imgs.box + 6 * first_img, nb, radsq, counts);
+    for (int q = 0; q < nb; q++) {
+      solid += counts[q];
+      const int sp = imgs.sph[first_img + q];
+      const int last = (q + 1 == nb) || imgs.sph[first_img + q + 1] != sp;
+      if (last && solid >= target) {
+        placed = sp + 1;
+        imgs.n = first_img + q + 1;
+        done = 1;
+        break;
+      }
+    }
+    if (npass < 64 && m < (1 << 20)) m *= 2;
+    else if (npass > 512 && m > 256) m /= 2;
+  }
+  free(pass);
+  free(counts);
+  slbd_rsa_destroy(D);
+  free(g.head); free(g.next); free(g.xyz);
+  /* the map of the placed spheres, then flattenobj over it and the objects
+   * already present */
+  unsigned char *map = (unsigned char *)slb_big_calloc(lb->nnodes, 1);
+  if (map == NULL) error("out of host memory");
+  slbd_sph_map(S->dev, imgs.o, imgs.box, imgs.n, radsq, map);
+  free(imgs.o); free(imgs.box); free(imgs.sph);
+  free_list(lb);
+  LBObj p = addobj(lb);
+  zeroobj(p);
+  LBReal ox = 0.0, oy = 0.0, oz = 0.0;
+  int ns = (int)merge_nodes(lb, p, 0, map, &ox, &oy, &oz);
+  slb_big_free(map);
+  if (ns == 0) {
+    if (slb_verbosity & OBJECT_DETAILS) printf("No sites set in object\n");
+    drop_head(lb, p);
+  } else {
+    lb->changed_objects = 1;
+    if (slb_verbosity & OBJECT_DETAILS)
+      printf("Object space flattened to id no %i, # nodes = %i\n", p->id, ns);
+    p->posx = ox / ns; p->posy = oy / ns; p->posz = oz / ns;
+  }
+  slb_obj_changed(lb);
+  if (phi_out) *phi_out = (double)solid / (double)lb->nnodes;
   return placed;
 }
 

@@ -50,7 +50,11 @@ def log(msg):
 
 # ---------------------------------------------------------------- 256^3 pack
 
-def test_pack256_tables_match_reference():
+@pytest.mark.parametrize("flat", [False, True])
+def test_pack256_tables_match_reference(flat):
+    """The 256^3 benchmark pack (slb_rsa_pack + flattenobj, or
+    slb_rsa_pack_flat with the sphere images counted and rasterised on the
+    GPU) against the reference library's tables of the same centres."""
     with gzip.open(os.path.join(GOLDEN, "pack256.json.gz"), "rt") as f:
         rec = json.load(f)
     import sunlightlb_amd as s
@@ -60,12 +64,15 @@ def test_pack256_tables_match_reference():
     cap = 4096
     cen = np.zeros(3 * cap)
     phi = C.c_double(0.0)
-    n = L.slb_rsa_pack(h, 8.0, 0.35, SEED, 50_000_000,
-                       cen.ctypes.data_as(C.POINTER(C.c_double)), cap, C.byref(phi))
+    pack = L.slb_rsa_pack_flat if flat else L.slb_rsa_pack
+    n = pack(h, 8.0, 0.35, SEED, 50_000_000, cen.ctypes.data_as(C.POINTER(C.c_double)), cap,
+             C.byref(phi))
     log(f"pack256: {n} spheres, phi {phi.value:.5f}")
     assert n == rec["spheres"]
     assert np.array_equal(cen[:3 * n].reshape(n, 3), np.array(rec["centres"]))
-    L.flattenobj(h)
+    assert phi.value == rec["solid"] / 256 ** 3
+    if not flat:
+        L.flattenobj(h)
     L.syncobjs(h)
     solid = lat.solid()
     assert int(solid.sum()) == rec["solid"]
@@ -74,6 +81,40 @@ def test_pack256_tables_match_reference():
     assert L.countfluid(h) == rec["countfluid"]
     assert L.countmask(h, b"links") == rec["countmask_links"]
     lat.free()
+
+
+@pytest.mark.parametrize("dims,R,phi", [((128, 96, 112), 6.0, 0.3), ((64, 64, 64), 9.0, 0.33)])
+def test_rsa_pack_flat_equals_pack_and_flatten(dims, R, phi):
+    """slb_rsa_pack_flat against slb_rsa_pack + flattenobj on a lattice that
+    already holds an object (a wall, merged into the flattened object by
+    both): same sphere count, centres, solid fraction, node map, object
+    position and next id."""
+    import sunlightlb_amd as s
+    L = lib()
+    out = []
+    for flat in (False, True):
+        lat = s.Lattice(*dims, precision=32)
+        h = lat.h
+        L.insertwall(h, b"y", 5)
+        cen = np.zeros(3 * 4096)
+        ph = C.c_double(0.0)
+        pack = L.slb_rsa_pack_flat if flat else L.slb_rsa_pack
+        n = pack(h, R, phi, SEED + 3, 5_000_000, cen.ctypes.data_as(C.POINTER(C.c_double)),
+                 4096, C.byref(ph))
+        if not flat:
+            L.flattenobj(h)
+        objs = lat.objects()
+        out.append((n, cen[:3 * n].copy(), ph.value, lat.solid(), len(objs),
+                    (objs[0].posx, objs[0].posy, objs[0].posz), lat.s.next_id))
+        lat.free()
+    a, b = out
+    assert a[0] == b[0] and a[0] > 10
+    assert np.array_equal(a[1], b[1])
+    assert a[2] == b[2]
+    assert np.array_equal(a[3], b[3])
+    assert a[4] == b[4] == 1
+    assert a[5] == b[5]
+    assert a[6] == b[6]
 
 
 def solve_pack256(L, prec):
