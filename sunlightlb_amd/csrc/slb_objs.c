@@ -143,20 +143,71 @@ LBObj insertlist(LBSys lb, char *listfile) {
   return p;
 }
 
+/* One object's full-lattice insertion pass on host threads: each range of
+ * nodes tests the shape's predicate, takes the nodes for object p and counts
+ * them and the overlaps (one object per pass, so the order of the ranges
+ * does not matter). */
+typedef struct {
+  LBSys lb;
+  LBObj p;
+  int shape;              /* 0 slab, 1 cylinder */
+  double b[6];            /* slab x1 x2 y1 y2 z1 z2; cylinder rad^2, a, b */
+  char axis, hole;        /* cylinder axis letter, hole when nonzero */
+  long ns[SLB_PAR_MAX];
+  int ov[SLB_PAR_MAX];
+} shape_pass;
+
+static void shape_range(void *c, long lo, long hi, int tid) {
+  shape_pass *J = (shape_pass *)c;
+  LBSys lb = J->lb;
+  const long esk = lb->esk, plane = (long)lb->esj * esk;
+  long ns = 0;
+  int ov = 0;
+  for (long n = lo; n < hi; n++) {
+    const int i = (int)(n / plane), r = (int)(n - (long)i * plane);
+    const int j = (int)(r / esk), k = (int)(r - (long)j * esk);
+    int within;
+    if (J->shape == 0) {
+      within = i > J->b[0] && i < J->b[1] && j > J->b[2] && j < J->b[3] && k > J->b[4] &&
+               k < J->b[5];
+    } else {
+      double u, v;
+      switch (J->axis) {
+        case 'x': case 'i': u = j - J->b[1]; v = k - J->b[2]; break;
+        case 'y': case 'j': u = i - J->b[1]; v = k - J->b[2]; break;
+        default: u = i - J->b[1]; v = j - J->b[2]; break;
+      }
+      const double rsq = u * u + v * v;
+      within = rsq < J->b[0];
+      if (J->hole) within = !within;
+    }
+    if (within) {
+      if (lb->obj[n] != NULL) ov = 1;
+      lb->obj[n] = J->p;
+      ns++;
+    }
+  }
+  J->ns[tid] = ns;
+  J->ov[tid] = ov;
+}
+
+static long shape_insert(shape_pass *J, int *overlap) {
+  const int nr = slb_par_for(J->lb->nnodes, shape_range, J);
+  long ns = 0;
+  for (int t = 0; t < nr; t++) { ns += J->ns[t]; if (J->ov[t]) *overlap = 1; }
+  return ns;
+}
+
 /* Open slab x1 < i < x2, y1 < j < y2, z1 < k < z2 (src/lbobjs.c:170-199). */
 LBObj insertslab(LBSys lb, LBReal x1, LBReal x2, LBReal y1, LBReal y2, LBReal z1, LBReal z2) {
   LBObj p = addobj(lb);
   zeroobj(p);
-  int ns = 0, overlap = 0;
-  for (int i = 0; i < lb->esi; i++)
-    for (int j = 0; j < lb->esj; j++)
-      for (int k = 0; k < lb->esk; k++)
-        if (i > x1 && i < x2 && j > y1 && j < y2 && k > z1 && k < z2) {
-          long n = nidx(lb, i, j, k);
-          if (lb->obj[n] != NULL) overlap = 1;
-          lb->obj[n] = p;
-          ns++;
-        }
+  int overlap = 0;
+  shape_pass J;
+  J.lb = lb; J.p = p; J.shape = 0;
+  J.b[0] = x1; J.b[1] = x2; J.b[2] = y1; J.b[3] = y2; J.b[4] = z1; J.b[5] = z2;
+  J.axis = 0; J.hole = 0;
+  int ns = (int)shape_insert(&J, &overlap);
   if (slb_verbosity & OBJECT_DETAILS) {
     printf("Inserted slab ");
     printf("%5.2f < x < %5.2f, ", x1, x2);
@@ -258,27 +309,14 @@ LBObj insertcyl(LBSys lb, char *type, LBReal rad, LBReal a, LBReal b) {
   }
   LBObj p = addobj(lb);
   zeroobj(p);
-  int ns = 0, overlap = 0;
-  double radsq = rad * rad;
-  for (int i = 0; i < lb->esi; i++)
-    for (int j = 0; j < lb->esj; j++)
-      for (int k = 0; k < lb->esk; k++) {
-        volatile double u, v;
-        switch (type[0]) {
-          case 'x': case 'i': u = j - a; v = k - b; break;
-          case 'y': case 'j': u = i - a; v = k - b; break;
-          default: u = i - a; v = j - b; break;
-        }
-        double rsq = u * u + v * v;
-        int within = (rsq < radsq);
-        if (type[1] != '\0') within = !within;
-        if (within) {
-          long n = nidx(lb, i, j, k);
-          if (lb->obj[n] != NULL) overlap = 1;
-          lb->obj[n] = p;
-          ns++;
-        }
-      }
+  int overlap = 0;
+  /* FP64 predicate (src/lbobjs.c:255-266): u^2 + v^2 < rad^2, this file
+   * built with -ffp-contract=off */
+  shape_pass J;
+  J.lb = lb; J.p = p; J.shape = 1;
+  J.b[0] = rad * rad; J.b[1] = a; J.b[2] = b;
+  J.axis = type[0]; J.hole = type[1] != '\0';
+  int ns = (int)shape_insert(&J, &overlap);
   if (slb_verbosity & OBJECT_DETAILS) {
     if (type[1] == '\0') printf("Cylinder "); else printf("Cylindrical hole ");
     printf("radius %5.2f, axis %5.2f %5.2f along %c ", rad, a, b, type[0]);

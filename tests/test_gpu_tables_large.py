@@ -2,7 +2,8 @@
 (552,960 nodes), checked against numpy restatements of the reference's
 rules: insertsph's FP64 predicate (i-ox)^2 + (j-oy)^2 + (k-oz)^2 < r^2 with
 the last insert winning (src/lbobjs.c:204-232), insertslab's open box
-(:170-199), makemask's link bits (:526-560), countfluid / countmask
+(:170-199), insertcyl's (i-a)^2 + (k-b)^2 < r^2 along y (:241-287),
+makemask's link bits (:526-560), countfluid / countmask
 (:648-714), flattenobj / invertobj centroids (:362-428) and zerobf / addbf
 on fluid nodes (:593-619)."""
 import numpy as np
@@ -19,7 +20,7 @@ C = np.array([[0, 0, 0], [1, 0, 0], [-1, 0, 0], [0, 1, 0], [0, -1, 0], [0, 0, 1]
 W = np.array([16, 8, 8, 8, 8, 8, 8, 1, 1, 1, 1, 1, 1, 1, 1])
 
 
-def raster(spheres, slab, dims):
+def raster(spheres, slab, cyl, dims):
     """Dense object index (list order: the newest object is 0), -1 fluid."""
     i, j, k = np.meshgrid(*[np.arange(n, dtype=np.float64) for n in dims], indexing="ij")
     owner = np.full(dims, -1, dtype=np.int64)
@@ -31,6 +32,10 @@ def raster(spheres, slab, dims):
             serial += 1
     x1, x2, y1, y2, z1, z2 = slab
     inside = (i > x1) & (i < x2) & (j > y1) & (j < y2) & (k > z1) & (k < z2)
+    owner[inside] = serial
+    serial += 1
+    r, a, b = cyl                   # along y: (i - a)^2 + (k - b)^2 < r^2
+    inside = (i - a) * (i - a) + (k - b) * (k - b) < r * r
     owner[inside] = serial
     serial += 1
     return np.where(owner < 0, -1, serial - 1 - owner).reshape(-1).astype(np.int32), serial
@@ -56,14 +61,16 @@ def test_tables_on_threads_match_restatement():
     spheres = [(float(rng.uniform(2.5, 9.0)), *(float(rng.uniform(-4, n + 4)) for n in DIMS))
                for _ in range(40)]
     slab = (10.5, 30.2, -1.0, 20.5, 40.3, 70.0)
+    cyl = (7.3, 52.6, 17.9)
     lat = lattice(DIMS)
     L, h = lat.L, lat.h
     assert lat.N > 1 << 18
     for sp in spheres:
         L.insertsph(h, *sp)
     L.insertslab(h, *slab)
+    L.insertcyl(h, b"y", *cyl)
     L.syncobjs(h)
-    want, nobj = raster(spheres, slab, DIMS)
+    want, nobj = raster(spheres, slab, cyl, DIMS)
     got = dense_objid(lat)
     assert (got == want).all()
     solid = want >= 0
