@@ -307,8 +307,8 @@ def mp_run(L, args, prec, n, R, phi, peclet, alpha, pads, baseline):
     mlups = N * args.mp_steps / (ms * 1e-3) / 1e6
     # algorithmic bytes: the 15 link factors of a node (its 15 pulled populations
     # of the frozen flow; the kernel reads them as 16 precomputed weights) and
-    # one 4-value node record {P, m1, m2, 1/rho} read and written
-    bpl = (15 + 2 * 4) * (4 if prec == 32 else 8)
+    # the node's {P, m1, m2} read and written
+    bpl = (15 + 2 * 3) * (4 if prec == 32 else 8)
     res = {
         "workload": f"{n}^3 RSA pack R {R:g} phi {phi_got:.4f} ({nsph} spheres), flow solved "
                     f"to residual {r.value:.2e} in {steps} steps, scaled to Pe {peclet:g} "

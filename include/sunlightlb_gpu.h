@@ -135,9 +135,9 @@ void slb_mp_track_moments(MPSys mp, int axis);
  * adsorbing sites, [3] sum fluid m1, [4] sum fluid m2, [5] fluid nodes. */
 void slb_mp_sums(MPSys mp, double out[6]);
 
-/* Run nsteps mpstep() calls without host round trips.  A run of 4 steps or
- * more computes each node's link factors from the frozen flow once and
- * reuses them (SLB_MP_WEIGHTS=0 keeps the per-step gather): the tracer a
+/* Run nsteps mpstep() calls without host round trips.  A run of 32 steps or
+ * more (SLB_MP_WEIGHTS=n sets the minimum, 0 never) computes each node's
+ * link factors from the frozen flow once and reuses them: the tracer a
  * partially adsorbing node keeps is then (sum of its link factors) x P
  * instead of the sum of the products, a rounding-level difference. */
 void slb_mp_run(MPSys mp, int nsteps);

@@ -904,19 +904,41 @@ k_mpweights(const T *__restrict__ a, const T *__restrict__ qin, T *__restrict__ 
     *(V4 *)(wt + wt_idx<I>(c, x)) = (V4){w[4 * c], w[4 * c + 1], w[4 * c + 2], w[4 * c + 3]};
 }
 
+/* Compact node records of a run on link factors: {P} or {P, m1, m2} without
+ * the 1/rho slot, which only the factors use (12 bytes per node FP32 with
+ * moments instead of 16). */
+template <typename T, int NM> struct CRec;
+template <typename T> struct __attribute__((aligned(sizeof(T)))) CRec<T, 3> { T p, m1, m2; };
+template <typename T> struct __attribute__((aligned(sizeof(T)))) CRec<T, 1> { T p; };
+
+template <typename T, int NM>
+__global__ void k_rec_pack(const T *__restrict__ q, int W, CRec<T, NM> *__restrict__ c, long N) {
+  const long x = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= N) return;
+  CRec<T, NM> r;
+  r.p = q[x * W];
+  if constexpr (NM > 1) { r.m1 = q[x * W + 1]; r.m2 = q[x * W + 2]; }
+  c[x] = r;
+}
+template <typename T, int NM>
+__global__ void k_rec_unpack(const CRec<T, NM> *__restrict__ c, T *__restrict__ q, int W, long N) {
+  const long x = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= N) return;
+  const CRec<T, NM> r = c[x];
+  q[x * W] = r.p;
+  if constexpr (NM > 1) { q[x * W + 1] = r.m1; q[x * W + 2] = r.m2; }
+}
+
 /* k_mpstep with the gather weights precomputed by k_mpweights: the same
- * sums over the same links in the same order. */
+ * sums over the same links in the same order, on compact records. */
 template <typename T, int NMOM, bool MOVING, typename I>
 __global__ void __launch_bounds__(256)
 __attribute__((amdgpu_waves_per_eu(1, sizeof(T) == 8 ? 4 : 8)))
-k_mpstep_w(const T *__restrict__ a, const T *__restrict__ wt, const T *__restrict__ qin,
-           T *__restrict__ qout, const uint32_t *__restrict__ mpf,
-           const int32_t *__restrict__ mask, const int32_t *__restrict__ objidx,
-           const double *__restrict__ tab, Geom G, MPArgs A) {
-  constexpr int W = NMOM > 1 ? 4 : 2;
-  typedef T V __attribute__((ext_vector_type(W)));
+k_mpstep_w(const T *__restrict__ wt, const CRec<T, NMOM> *__restrict__ Q,
+           CRec<T, NMOM> *__restrict__ qout, const uint32_t *__restrict__ mpf,
+           const int32_t *__restrict__ mask, Geom G, MPArgs A) {
   typedef T V4 __attribute__((ext_vector_type(4)));
-  const V *__restrict__ Q = (const V *)qin;
+  typedef CRec<T, NMOM> R;
   Geom Gb = G;
   Gb.xcd_map = G.mp_xcd;
   int blk_x, blk_y;
@@ -929,9 +951,7 @@ k_mpstep_w(const T *__restrict__ a, const T *__restrict__ wt, const T *__restric
   const AxisOffT<I> oj = axis_off_t<I>(j, G.esj, (I)G.esk);
   const AxisOffT<I> ok = axis_off_t<I>(k, G.esk, (I)1);
   const uint32_t f = mpf[x];
-  const T r0 = T(A.rho0), al = T(A.alpha);
-  const V qx = Q[x];
-  const T rix = qx[W - 1];
+  const R qx = Q[x];
   T wv[16];
   {
 #pragma unroll
@@ -940,7 +960,7 @@ k_mpstep_w(const T *__restrict__ a, const T *__restrict__ wt, const T *__restric
       wv[4 * c] = v[0]; wv[4 * c + 1] = v[1]; wv[4 * c + 2] = v[2]; wv[4 * c + 3] = v[3];
     }
   }
-  V qv[15];
+  R qv[15];
   Links<1, 15>::run([&](auto LL) {
     constexpr int l = decltype(LL)::value;
     qv[l] = Q[x + nb_off<l, -1>(oi, oj, ok)];
@@ -948,45 +968,44 @@ k_mpstep_w(const T *__restrict__ a, const T *__restrict__ wt, const T *__restric
   T np = T(0), n1 = T(0), n2 = T(0);
   if (f & 1u) {
     const T pr = wv[0];
-    np += pr * qx[0];
-    if (NMOM > 1) { n1 += pr * qx[1]; n2 += pr * qx[2]; }
+    np += pr * qx.p;
+    if constexpr (NMOM > 1) { n1 += pr * qx.m1; n2 += pr * qx.m2; }
   }
   Links<1, 15>::run([&](auto LL) {
     constexpr int l = decltype(LL)::value;
-    const V qy = qv[l];
-    const T P = qy[0];
+    const R qy = qv[l];
+    const T P = qy.p;
     const T pr = wv[l];
     np += pr * P;
-    if (NMOM > 1) {
+    if constexpr (NMOM > 1) {
       const int dd = (A.axis == 0) ? CX(l) : (A.axis == 1) ? CY(l) : CZ(l);
-      const T q1 = qy[1];
+      const T q1 = qy.m1;
       n1 += pr * (q1 + T(dd) * P);
-      n2 += pr * (qy[2] + T(2 * dd) * q1 + T(dd * dd) * P);
+      n2 += pr * (qy.m2 + T(2 * dd) * q1 + T(dd * dd) * P);
     }
   });
   if (f >> 17) {
-    const T R = wv[15];
-    np += R * qx[0];
-    if (NMOM > 1) { n1 += R * qx[1]; n2 += R * qx[2]; }
+    const T Rk = wv[15];
+    np += Rk * qx.p;
+    if constexpr (NMOM > 1) { n1 += Rk * qx.m1; n2 += Rk * qx.m2; }
   }
-  V o;
-  o[W - 1] = rix;
-  o[0] = T(0);
-  if (NMOM > 1) { o[1] = T(0); o[2] = T(0); }
+  R o;
+  o.p = T(0);
+  if constexpr (NMOM > 1) { o.m1 = T(0); o.m2 = T(0); }
   if (f & 1u) {
-    o[0] = np;
-    if (NMOM > 1) { o[1] = n1; o[2] = n2; }
+    o.p = np;
+    if constexpr (NMOM > 1) { o.m1 = n1; o.m2 = n2; }
   } else {
     const int m = mask[x];
     T frac = T(0);
     if (m & 0x4000) frac = T(1);
     else if (m != 0 && A.pads > 0.0) frac = T(A.pads);
     if (frac != T(0)) {
-      o[0] = qx[0] + frac * np;
-      if (NMOM > 1) { o[1] = qx[1] + frac * n1; o[2] = qx[2] + frac * n2; }
+      o.p = qx.p + frac * np;
+      if constexpr (NMOM > 1) { o.m1 = qx.m1 + frac * n1; o.m2 = qx.m2 + frac * n2; }
     }
   }
-  ((V *)qout)[x] = o;
+  qout[x] = o;
 }
 
 /* record component access: q[x * W + c] */
@@ -1092,6 +1111,7 @@ struct slbd_mp {
   long N;
   long rho_version;  /* lattice density version the 1/rho slots hold */
   void *wt;        /* gather weights of the current run (16 per node, tiled), or NULL */
+  void *cq[2];     /* compact records of a run on the weights, or NULL */
 };
 
 static size_t tsize(int prec) { return prec == 32 ? sizeof(float) : sizeof(double); }
@@ -2096,6 +2116,8 @@ extern "C" void slbd_mp_destroy(slbd_mp *M) {
   for (int b = 0; b < 2; b++)
     if (M->q[b]) HIPCHECK(hipFree(M->q[b]));
   if (M->wt) HIPCHECK(hipFree(M->wt));
+  for (int b = 0; b < 2; b++)
+    if (M->cq[b]) HIPCHECK(hipFree(M->cq[b]));
   free(M);
 }
 
@@ -2200,12 +2222,10 @@ extern "C" void slbd_mp_zero(slbd_lattice *L, slbd_mp *M, int field) {
 /* MP steps of one run with precomputed gather weights: runs of at least
  * this many steps (SLB_MP_WEIGHTS=0 turns the weights off) */
 static int mp_weights_min(void) {
-  static int v = -1;
-  if (v < 0) {
-    const char *e = getenv("SLB_MP_WEIGHTS");
-    v = (e && atoi(e) == 0) ? 0 : 4;
-  }
-  return v;
+  /* the factor pass and the record packing cost about four steps; read per
+   * call so that a caller can change it (0: never) */
+  const char *e = getenv("SLB_MP_WEIGHTS");
+  return e ? atoi(e) : 32;
 }
 
 template <typename T, int NMOM, bool MOVING>
@@ -2220,21 +2240,30 @@ static void launch_mp_w(slbd_lattice *L, slbd_mp *M, const MPArgs &A, int nsteps
                        (const T *)L->pops[L->cur], (const T *)M->q[M->cur], (T *)M->wt, L->mpf,
                        L->objidx, L->tab, M->W, L->G, A);
   LAUNCHCHECK();
+  /* the run on compact records: packed from the current records, unpacked
+   * back into them at the end (their 1/rho slots stay as they are) */
+  typedef CRec<T, NMOM> R;
+  for (int b = 0; b < 2; b++)
+    if (!M->cq[b]) M->cq[b] = dalloc((size_t)L->N * sizeof(R));
+  hipLaunchKernelGGL((k_rec_pack<T, NMOM>), lin_grid(L->N), dim3(256), 0, L->stream,
+                     (const T *)M->q[M->cur], M->W, (R *)M->cq[0], L->N);
+  LAUNCHCHECK();
+  int c = 0;
   for (int s = 0; s < nsteps; s++) {
-    const int c = M->cur;
-    if (!MOVING && L->idx32)
+    if (L->idx32)
       hipLaunchKernelGGL((k_mpstep_w<T, NMOM, MOVING, int>), plane_grid(L), dim3(256), 0,
-                         L->stream, (const T *)L->pops[L->cur], (const T *)M->wt,
-                         (const T *)M->q[c], (T *)M->q[c ^ 1], L->mpf, L->mask, L->objidx,
-                         L->tab, L->G, A);
+                         L->stream, (const T *)M->wt, (const R *)M->cq[c], (R *)M->cq[c ^ 1],
+                         L->mpf, L->mask, L->G, A);
     else
       hipLaunchKernelGGL((k_mpstep_w<T, NMOM, MOVING, long>), plane_grid(L), dim3(256), 0,
-                         L->stream, (const T *)L->pops[L->cur], (const T *)M->wt,
-                         (const T *)M->q[c], (T *)M->q[c ^ 1], L->mpf, L->mask, L->objidx,
-                         L->tab, L->G, A);
+                         L->stream, (const T *)M->wt, (const R *)M->cq[c], (R *)M->cq[c ^ 1],
+                         L->mpf, L->mask, L->G, A);
     LAUNCHCHECK();
-    M->cur ^= 1;
+    c ^= 1;
   }
+  hipLaunchKernelGGL((k_rec_unpack<T, NMOM>), lin_grid(L->N), dim3(256), 0, L->stream,
+                     (const R *)M->cq[c], (T *)M->q[M->cur], M->W, L->N);
+  LAUNCHCHECK();
 }
 
 template <typename T, int NMOM, bool MOVING>

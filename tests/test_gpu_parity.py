@@ -133,12 +133,14 @@ def test_moving_walls_match_reference():
     assert rel_l2(vm, g["meas_vmean"]) < TOL64
 
 
-@pytest.mark.parametrize("stepping", ["mpstep", "run"])
+@pytest.mark.parametrize("stepping", ["mpstep", "run", "run_factors"])
 @pytest.mark.parametrize("variant", ["reflect", "adsorb", "moving"])
-def test_mp_matches_reference(variant, stepping):
+def test_mp_matches_reference(variant, stepping, monkeypatch):
     """25 MP steps against the reference library's fields: one mpstep call
-    per step, or one slb_mp_run call (whose steps share gather weights
-    computed once from the frozen flow)."""
+    per step, one slb_mp_run call gathering pulled populations, or one whose
+    steps share link factors computed once from the frozen flow and run on
+    compact records (SLB_MP_WEIGHTS=1 forces that form for any run)."""
+    monkeypatch.setenv("SLB_MP_WEIGHTS", "1" if stepping == "run_factors" else "0")
     g = load_golden("mp.npz")
     dims = tuple(g["meta"]["dims"])
     lat = lattice(dims)
@@ -166,7 +168,7 @@ def test_mp_matches_reference(variant, stepping):
         for _ in range(25):
             L.mpstep(mp)
     else:
-        L.slb_mp_run(mp, 25)
+        L.slb_mp_run(mp, 25)        # "run" and "run_factors"
     L.slb_mp_sync_host(mp)
     import sunlightlb_amd as s
     p = s.view(mp.contents.p, lat.N).copy()
@@ -266,7 +268,12 @@ def test_against_live_oracle_ns_moving_skip(oracle):
     assert rel_l2(vm, res["vmean"]) < TOL64
 
 
-def test_mp_partial_adsorption_and_moments_against_oracle(oracle):
+@pytest.mark.parametrize("factors", ["0", "1"])
+def test_mp_partial_adsorption_and_moments_against_oracle(oracle, factors, monkeypatch):
+    """Partial adsorption 0.2 and displacement moments, 20 steps in one run:
+    the gather of pulled populations (SLB_MP_WEIGHTS=0) and the link factors
+    on compact records (SLB_MP_WEIGHTS=1) against the CPU oracle."""
+    monkeypatch.setenv("SLB_MP_WEIGHTS", factors)
     g = load_golden("mp.npz")
     dims = tuple(g["meta"]["dims"])
     lat = lattice(dims)

@@ -175,10 +175,13 @@ def mp_setup(L, prec, pads=0.01, peclet=10.0):
     return h, mp
 
 
-def test_mp128_matches_oracle_at_full_size(oracle):
+@pytest.mark.parametrize("factors", ["0", "1"])
+def test_mp128_matches_oracle_at_full_size(oracle, factors, monkeypatch):
     """20 moment-propagation steps at 128^3 (src/mpkernel.c:47-368 with the
     partial-adsorption rule and displacement moments) against the CPU
-    oracle on the same populations, density and tables: FP64, 1e-12."""
+    oracle on the same populations, density and tables: FP64, 1e-12; the
+    gather of pulled populations and the link factors on compact records."""
+    monkeypatch.setenv("SLB_MP_WEIGHTS", factors)
     import sunlightlb_amd as s
     from test_gpu_parity import _dense_objid_fast
     L = lib()
