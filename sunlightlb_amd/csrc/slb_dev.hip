@@ -931,7 +931,7 @@ __global__ void k_rec_unpack(const CRec<T, NM> *__restrict__ c, T *__restrict__ 
 
 /* k_mpstep with the gather weights precomputed by k_mpweights: the same
  * sums over the same links in the same order, on compact records. */
-template <typename T, int NMOM, bool MOVING, typename I>
+template <typename T, int NMOM, typename I>
 __global__ void __launch_bounds__(256)
 __attribute__((amdgpu_waves_per_eu(1, sizeof(T) == 8 ? 4 : 8)))
 k_mpstep_w(const T *__restrict__ wt, const CRec<T, NMOM> *__restrict__ Q,
@@ -2251,11 +2251,11 @@ static void launch_mp_w(slbd_lattice *L, slbd_mp *M, const MPArgs &A, int nsteps
   int c = 0;
   for (int s = 0; s < nsteps; s++) {
     if (L->idx32)
-      hipLaunchKernelGGL((k_mpstep_w<T, NMOM, MOVING, int>), plane_grid(L), dim3(256), 0,
+      hipLaunchKernelGGL((k_mpstep_w<T, NMOM, int>), plane_grid(L), dim3(256), 0,
                          L->stream, (const T *)M->wt, (const R *)M->cq[c], (R *)M->cq[c ^ 1],
                          L->mpf, L->mask, L->G, A);
     else
-      hipLaunchKernelGGL((k_mpstep_w<T, NMOM, MOVING, long>), plane_grid(L), dim3(256), 0,
+      hipLaunchKernelGGL((k_mpstep_w<T, NMOM, long>), plane_grid(L), dim3(256), 0,
                          L->stream, (const T *)M->wt, (const R *)M->cq[c], (R *)M->cq[c ^ 1],
                          L->mpf, L->mask, L->G, A);
     LAUNCHCHECK();
