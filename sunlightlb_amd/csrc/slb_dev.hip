@@ -838,19 +838,26 @@ k_mpstep(const T *__restrict__ a, const T *__restrict__ qin, T *__restrict__ qou
  * carries from its source y = x - c_l,
  *   pr_l(x) = a_l(y) / rho(y) + (w_l / 72) (rho0 / rho(y) - alpha)
  * (zero when y is solid; the rest link adds alpha), is the same at every
- * step, and so is the sum over the partially adsorbing links of x of the
- * factor its own tracer keeps (slot 15).  k_mpweights stores the 16 of
- * each node; k_mpstep_w then reads them as four vector loads instead of 15
- * pulled population loads (and the own-node loads of partially adsorbing
- * links), and gathers only the records.  The one difference in arithmetic:
- * the kept own tracer is (sum of the link factors) x P rather than the sum
- * of the products. */
-/* weights in tiles of 1024 nodes: four runs (weights 4c .. 4c+3) of 1024
- * four-value vectors, so a wave's load of one run is 1 KiB contiguous */
+ * step, and so is the factor of everything a node keeps of its own tracer:
+ * the rest link's and the sum over its partially adsorbing links.
+ * k_mpweights stores the 15 factors of each node (slot 0 the own one);
+ * k_mpstep_w then reads them as four vector loads instead of 15 pulled
+ * population loads (and the own-node loads of partially adsorbing links),
+ * and gathers only the records.  The one difference in arithmetic: the
+ * kept own tracer is (sum of the own factors) x P rather than the sum of
+ * the products. */
+/* weights in tiles of 1024 nodes: runs of 1024 four-value vectors (weights
+ * 0-3, 4-7, 8-11) and one of 1024 three-value vectors (12-14), so a wave's
+ * load of one run is contiguous */
 template <typename I>
 __device__ __forceinline__ I wt_idx(int c, I x) {
-  return (((x >> POP_SHIFT) * I(4) + I(c)) * I(POP_TILE) + (x & I(POP_TILE - 1))) * I(4);
+  const I tile = (x >> POP_SHIFT) * I(15 * POP_TILE), n = x & I(POP_TILE - 1);
+  return c < 3 ? tile + (I(c) * I(POP_TILE) + n) * I(4) : tile + I(12 * POP_TILE) + n * I(3);
 }
+template <typename T>
+struct __attribute__((aligned(sizeof(T)))) Wt3 {
+  T a, b, c;
+};
 
 template <typename T, bool MOVING, typename I>
 __global__ void __launch_bounds__(256)
@@ -897,11 +904,12 @@ k_mpweights(const T *__restrict__ a, const T *__restrict__ qin, T *__restrict__ 
       }
     });
   }
-  w[15] = R;
+  w[0] += R;      /* zero where the node keeps nothing of its own */
   typedef T V4 __attribute__((ext_vector_type(4)));
 #pragma unroll
-  for (int c = 0; c < 4; c++)
+  for (int c = 0; c < 3; c++)
     *(V4 *)(wt + wt_idx<I>(c, x)) = (V4){w[4 * c], w[4 * c + 1], w[4 * c + 2], w[4 * c + 3]};
+  *(Wt3<T> *)(wt + wt_idx<I>(3, x)) = Wt3<T>{w[12], w[13], w[14]};
 }
 
 /* Compact node records of a run on link factors: {P} or {P, m1, m2} without
@@ -952,13 +960,15 @@ k_mpstep_w(const T *__restrict__ wt, const CRec<T, NMOM> *__restrict__ Q,
   const AxisOffT<I> ok = axis_off_t<I>(k, G.esk, (I)1);
   const uint32_t f = mpf[x];
   const R qx = Q[x];
-  T wv[16];
+  T wv[15];
   {
 #pragma unroll
-    for (int c = 0; c < 4; c++) {
+    for (int c = 0; c < 3; c++) {
       const V4 v = *(const V4 *)(wt + wt_idx<I>(c, x));
       wv[4 * c] = v[0]; wv[4 * c + 1] = v[1]; wv[4 * c + 2] = v[2]; wv[4 * c + 3] = v[3];
     }
+    const Wt3<T> v = *(const Wt3<T> *)(wt + wt_idx<I>(3, x));
+    wv[12] = v.a; wv[13] = v.b; wv[14] = v.c;
   }
   R qv[15];
   Links<1, 15>::run([&](auto LL) {
@@ -966,7 +976,7 @@ k_mpstep_w(const T *__restrict__ wt, const CRec<T, NMOM> *__restrict__ Q,
     qv[l] = Q[x + nb_off<l, -1>(oi, oj, ok)];
   });
   T np = T(0), n1 = T(0), n2 = T(0);
-  if (f & 1u) {
+  if ((f & 1u) || (f >> 17)) {
     const T pr = wv[0];
     np += pr * qx.p;
     if constexpr (NMOM > 1) { n1 += pr * qx.m1; n2 += pr * qx.m2; }
@@ -984,11 +994,6 @@ k_mpstep_w(const T *__restrict__ wt, const CRec<T, NMOM> *__restrict__ Q,
       n2 += pr * (qy.m2 + T(2 * dd) * q1 + T(dd * dd) * P);
     }
   });
-  if (f >> 17) {
-    const T Rk = wv[15];
-    np += Rk * qx.p;
-    if constexpr (NMOM > 1) { n1 += Rk * qx.m1; n2 += Rk * qx.m2; }
-  }
   R o;
   o.p = T(0);
   if constexpr (NMOM > 1) { o.m1 = T(0); o.m2 = T(0); }
@@ -1110,7 +1115,7 @@ struct slbd_mp {
   int cur, nmom, prec, W;
   long N;
   long rho_version;  /* lattice density version the 1/rho slots hold */
-  void *wt;        /* gather weights of the current run (16 per node, tiled), or NULL */
+  void *wt;        /* gather weights of the current run (15 per node, tiled), or NULL */
   void *cq[2];     /* compact records of a run on the weights, or NULL */
 };
 
@@ -2230,7 +2235,7 @@ static int mp_weights_min(void) {
 
 template <typename T, int NMOM, bool MOVING>
 static void launch_mp_w(slbd_lattice *L, slbd_mp *M, const MPArgs &A, int nsteps) {
-  if (!M->wt) M->wt = dalloc((size_t)pop_nodes(L->N) * 16 * sizeof(T));
+  if (!M->wt) M->wt = dalloc((size_t)pop_nodes(L->N) * 15 * sizeof(T));
   if (L->idx32)
     hipLaunchKernelGGL((k_mpweights<T, MOVING, int>), plane_grid(L), dim3(256), 0, L->stream,
                        (const T *)L->pops[L->cur], (const T *)M->q[M->cur], (T *)M->wt, L->mpf,
