@@ -305,9 +305,9 @@ def mp_run(L, args, prec, n, R, phi, peclet, alpha, pads, baseline):
     var = s1[4] / ptot - mean * mean
     N = n ** 3
     mlups = N * args.mp_steps / (ms * 1e-3) / 1e6
-    # algorithmic bytes: the 15 link factors of a node (its 15 pulled populations
-    # of the frozen flow; the kernel reads them as 16 precomputed weights) and
-    # the node's {P, m1, m2} read and written
+    # algorithmic bytes: the 15 link factors of a node (from its 15 pulled
+    # populations of the frozen flow, precomputed once per run) and the node's
+    # {P, m1, m2} read and written
     bpl = (15 + 2 * 3) * (4 if prec == 32 else 8)
     res = {
         "workload": f"{n}^3 RSA pack R {R:g} phi {phi_got:.4f} ({nsph} spheres), flow solved "

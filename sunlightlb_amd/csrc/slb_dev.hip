@@ -194,6 +194,10 @@ k_step_lds(const T *__restrict__ src, T *__restrict__ dst,
   const unsigned fl = flags[x];
   const bool solid = (fl & FLAG_SOLID) != 0;
   T v[15];
+  /* FP32: the wave's loads issue at raised priority until the barrier
+   * (357.6 against 359.6 us per step at 256^3; FP64 measured 0.8% slower
+   * with it, so it stays at the default there) */
+  if (sizeof(T) == 4) __builtin_amdgcn_s_setprio(3);
   /* stage: the source row of each cz != 0 link, aligned (no k shift) */
   Links<5, 15>::run([&](auto LL) {
     constexpr int l = decltype(LL)::value;
@@ -211,6 +215,7 @@ k_step_lds(const T *__restrict__ src, T *__restrict__ dst,
     own[l] = ((fl >> (l - 1)) & 1u) ? src[pidx_t<I>(OPP(l), x)] : T(0);
   });
   __syncthreads();
+  if (sizeof(T) == 4) __builtin_amdgcn_s_setprio(0);
   const int rbase = threadIdx.x - k;      /* first thread of this row in the block */
   Links<5, 15>::run([&](auto LL) {
     constexpr int l = decltype(LL)::value;
