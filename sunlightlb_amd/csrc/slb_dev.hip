@@ -267,6 +267,114 @@ k_step_lds(const T *__restrict__ src, T *__restrict__ dst,
   }
 }
 
+/* The staged step for rows of any length esk <= 512: a block holds rpb
+ * whole rows (rpb divides esj, rpb * esk <= 512) and as many threads,
+ * rounded up to whole wavefronts; threads past rpb * esk only take part in
+ * the barrier.  The staging and the wrap of k - cz within the row are those
+ * of k_step_lds with the row length a runtime value (compares instead of a
+ * mask), the LDS sized to the block (10 values per thread).  Same values,
+ * same arithmetic: bit-identical to k_step. */
+template <typename T, bool MOVING, bool FORCE, int BF, typename I>
+__global__ void __launch_bounds__(512)
+k_step_ldsr(const T *__restrict__ src, T *__restrict__ dst,
+            const uint16_t *__restrict__ flags,
+            const double *__restrict__ ux, const double *__restrict__ uy,
+            const double *__restrict__ uz,
+            const T *__restrict__ bx, const T *__restrict__ by, const T *__restrict__ bz,
+            double *__restrict__ fx, double *__restrict__ fy, double *__restrict__ fz,
+            Geom G, PhysArgs P, int rpb) {
+  extern __shared__ __attribute__((aligned(16))) unsigned char ldsr_raw[];
+  T *row = (T *)ldsr_raw;                 /* [10][blockDim.x] */
+  const int bs = blockDim.x;
+  int blk_x, blk_y;
+  step_block(G, blk_x, blk_y);
+  const int esk = G.esk;
+  const int r = threadIdx.x / esk;        /* row within the block */
+  const int k = threadIdx.x - r * esk;
+  const int j = blk_x * rpb + r;
+  const bool act = r < rpb;               /* rpb divides esj: j < esj */
+  const int i = blk_y;
+  const I x = (I)i * (I)G.plane + (I)j * (I)esk + (I)k;
+  const AxisOffT<I> oi = axis_off_t<I>(i, G.esi, (I)G.plane);
+  const AxisOffT<I> oj = axis_off_t<I>(act ? j : 0, G.esj, (I)esk);
+  const AxisOffT<I> ok = axis_off_t<I>(k, esk, (I)1);
+  const AxisOffT<I> o0 = {0, 0};
+  const unsigned fl = act ? flags[x] : 0u;
+  const bool solid = (fl & FLAG_SOLID) != 0;
+  T v[15];
+  T own[15];
+  if (act) {
+    Links<5, 15>::run([&](auto LL) {
+      constexpr int l = decltype(LL)::value;
+      row[(l - 5) * bs + threadIdx.x] = src[pidx_t<I>(l, x + nb_off<l, -1>(oi, oj, o0))];
+    });
+    v[0] = src[pidx_t<I>(0, x)];
+    Links<1, 5>::run([&](auto LL) {
+      constexpr int l = decltype(LL)::value;
+      const bool bb = (fl >> (l - 1)) & 1u;
+      v[l] = src[bb ? pidx_t<I>(OPP(l), x) : pidx_t<I>(l, x + nb_off<l, -1>(oi, oj, ok))];
+    });
+    Links<5, 15>::run([&](auto LL) {
+      constexpr int l = decltype(LL)::value;
+      own[l] = ((fl >> (l - 1)) & 1u) ? src[pidx_t<I>(OPP(l), x)] : T(0);
+    });
+  }
+  __syncthreads();
+  if (!act) return;
+  const int rbase = threadIdx.x - k;
+  Links<5, 15>::run([&](auto LL) {
+    constexpr int l = decltype(LL)::value;
+    int kw = k - CZ(l);
+    kw += (kw < 0) ? esk : 0;
+    kw -= (kw >= esk) ? esk : 0;
+    const T pulled = row[(l - 5) * bs + rbase + kw];
+    v[l] = ((fl >> (l - 1)) & 1u) ? own[l] : pulled;
+    if (FORCE) own[l] = pulled;
+  });
+  if ((MOVING || FORCE) && (fl & FLAG_LINKS)) {
+    double Fx = 0.0, Fy = 0.0, Fz = 0.0;
+    Links<1, 15>::run([&](auto LL) {
+      constexpr int l = decltype(LL)::value;
+      if ((fl >> (l - 1)) & 1u) {
+        double cu = 0.0;
+        if (MOVING) {
+          const I s = solid ? x : x + nb_off<l, -1>(oi, oj, ok);
+          cu = (double)WL(l) * ((CX(l) ? CX(l) * ux[s] : 0.0) + (CY(l) ? CY(l) * uy[s] : 0.0) +
+                                (CZ(l) ? CZ(l) * uz[s] : 0.0));
+        }
+        if (FORCE && solid) {
+          const double a = (double)v[l];
+          const double b = CZ(l) ? (double)own[l]
+                                 : (double)src[pidx_t<I>(l, x + nb_off<l, -1>(oi, oj, ok))];
+          const double g = -2.0 * (a - b + cu);
+          if (CX(l)) Fx += CX(l) * g;
+          if (CY(l)) Fy += CY(l) * g;
+          if (CZ(l)) Fz += CZ(l) * g;
+        }
+        if (MOVING) v[l] = (T)((double)v[l] + cu);
+      }
+    });
+    if (FORCE && solid) {
+      fx[x] += Fx; fy[x] += Fy; fz[x] += Fz;
+    }
+  }
+  if (!(P.skip_solid && solid)) {
+    T jbx = T(0), jby = T(0), jbz = T(0);
+    if (BF != 0 && !solid) {
+      jbx = T(P.bfx); jby = T(P.bfy); jbz = T(P.bfz);
+      if (BF == 2) { jbx += bx[x]; jby += by[x]; jbz += bz[x]; }
+    }
+    collide_node<T>(v, jbx, jby, jbz, P);
+  }
+  if (G.nt_store) {
+#pragma unroll
+    for (int l = 0; l < 15; l++) __builtin_nontemporal_store(v[l], &dst[pidx_t<I>(l, x)]);
+  } else {
+#pragma unroll
+    for (int l = 0; l < 15; l++) dst[pidx_t<I>(l, x)] = v[l];
+  }
+}
+
 /* ---------------------------------------------------------------------- */
 /* Separate phases (bbl, propagate, collide) on the live buffer.           */
 
@@ -1111,6 +1219,7 @@ struct slbd_lattice {
   hipEvent_t ev0, ev1;
   int sweep_nt_mid;         /* two-step sweep: non-temporal window stores */
   int step_lds;             /* step with the k pulls staged in LDS (k_step_lds) */
+  int ldsr_rpb, ldsr_bs;    /* k_step_ldsr: rows per block, threads per block (0: off) */
   void *sph_buf;            /* sphere images of a pack being placed (grow-only) */
   size_t sph_cap;
 };
@@ -1206,6 +1315,32 @@ extern "C" slbd_lattice *slbd_create(int esi, int esj, int esk, int prec) {
     const int e = esk;
     L->step_lds = want && (e == 64 || e == 128 || e == 256 || e == 512) &&
                   (L->G.plane % 512) == 0;
+    /* other rows of up to 512 nodes: k_step_ldsr with the most whole rows
+     * that divide the plane's rows and fit 512 threads, where at least 85%
+     * of the block's threads hold a node, on lattices of 2^18 nodes and
+     * more, and where either every block starts on a 128-byte line or both
+     * population buffers fit the last-level cache.  Measured FP32 against
+     * k_step: 80^3 13.2/15.0 us, 96^3 20.1/23.5, 100^3 26.4/27.2, 120^3
+     * 42.3/43.6, 160^3 94.0/99.0 (FP64 96^3 38.6/43.2, 160^3 185/188); but
+     * rows whose blocks start off the line boundary, streamed from HBM, are
+     * slower (200^3 212/181, 300^3 724/646 us), as are lattices too small
+     * to fill the GPU (32^3 4.7/4.5, 48^3 7.0/6.1 us).  SLB_STEP_LDS=2 uses
+     * it wherever rows fit (for tests). */
+    L->ldsr_rpb = L->ldsr_bs = 0;
+    if (want && !L->step_lds && e <= 512) {
+      int rpb = 1;
+      for (int q = 512 / e; q >= 1; q--)
+        if (esj % q == 0) { rpb = q; break; }
+      const int n = rpb * e, bs = (n + 63) & ~63;
+      const long ts = (long)tsize(prec);
+      const bool aligned = ((long)n * ts) % 128 == 0 && ((long)L->G.plane * ts) % 128 == 0;
+      const bool resident = 30.0 * (double)L->N * (double)ts <= 192e6;
+      if (want == 2 || (bs >= 128 && 100 * n >= 85 * bs && L->N >= (1L << 18) &&
+                        (aligned || resident))) {
+        L->ldsr_rpb = rpb;
+        L->ldsr_bs = bs;
+      }
+    }
   }
   {
     /* MP kernel: 256-thread blocks over (row block, plane) */
@@ -1730,6 +1865,25 @@ static void launch_step_t(slbd_lattice *L, const PhysArgs &A, const void *srcv, 
       hipLaunchKernelGGL((k_step_lds<T, MOVING, FORCE, BF, long>), g2, dim3(512), 0, L->stream,
                          src, dst, L->flags, L->ux, L->uy, L->uz, (const T *)L->bx,
                          (const T *)L->by, (const T *)L->bz, L->fx, L->fy, L->fz, G, A);
+    LAUNCHCHECK();
+    return;
+  }
+  if (L->ldsr_rpb && i0 == 0 && np == L->esi && sring == 0 && dring == 0) {
+    /* the tile orders (xcd_map >= 2) were chosen for k_step's grid */
+    Geom Gr = G;
+    if (Gr.xcd_map != 1) Gr.xcd_map = 0;
+    const dim3 g2((unsigned)(G.esj / L->ldsr_rpb), (unsigned)np);
+    const size_t sh = (size_t)10 * (size_t)L->ldsr_bs * sizeof(T);
+    if (L->idx32)
+      hipLaunchKernelGGL((k_step_ldsr<T, MOVING, FORCE, BF, int>), g2, dim3(L->ldsr_bs), sh,
+                         L->stream, src, dst, L->flags, L->ux, L->uy, L->uz, (const T *)L->bx,
+                         (const T *)L->by, (const T *)L->bz, L->fx, L->fy, L->fz, Gr, A,
+                         L->ldsr_rpb);
+    else
+      hipLaunchKernelGGL((k_step_ldsr<T, MOVING, FORCE, BF, long>), g2, dim3(L->ldsr_bs), sh,
+                         L->stream, src, dst, L->flags, L->ux, L->uy, L->uz, (const T *)L->bx,
+                         (const T *)L->by, (const T *)L->bz, L->fx, L->fy, L->fz, Gr, A,
+                         L->ldsr_rpb);
     LAUNCHCHECK();
     return;
   }

@@ -143,3 +143,50 @@ def test_lds_staged_step_bit_identical(prec, dims, monkeypatch):
     ref, got = run("0"), run("1")
     for r, g in zip(ref, got):
         assert np.array_equal(r, g)
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+@pytest.mark.parametrize("dims", [(5, 96, 96), (3, 10, 100), (2, 6, 160), (4, 8, 200), (7, 5, 48),
+                                  (3, 7, 97), (2, 3, 33), (2, 4, 64), (2, 5, 300), (6, 32, 32),
+                                  (2, 3, 1), (1, 1, 7), (2, 2, 2), (2, 1, 3)])
+def test_row_staged_step_bit_identical(prec, dims, monkeypatch):
+    """k_step_ldsr (the k pulls staged in LDS for rows of any length up to
+    512 nodes, as many whole rows per block as divide the plane; forced on
+    every shape with SLB_STEP_LDS=2, including rows of 1-3 nodes whose k
+    pulls wrap onto the node itself) against k_step: populations after odd
+    and even step counts and the node forces of measure(), moving walls,
+    Navier-Stokes, per-node body force."""
+    import sunlightlb_amd as s
+
+    def run(mode):
+        monkeypatch.setenv("SLB_STEP_LDS", mode)
+        lat = s.Lattice(*dims, precision=prec)
+        L, h = lat.L, lat.h
+        lat.s.tau, lat.s.stokes_flow, lat.s.objects_moving = 0.7, 0, 1
+        rng = np.random.default_rng(sum(dims))
+        for q in range(6):
+            r = float(rng.uniform(0.6, 3.5))
+            c = [float(rng.uniform(-1.0, n)) for n in dims]
+            sph = L.insertsph(h, r, *c)
+            if sph:
+                sph.contents.velx, sph.contents.vely = 0.01 * (q + 1), -0.004
+                sph.contents.velz, sph.contents.angz = 0.006, 0.002
+        L.vspread(h)
+        L.zerobf(h)
+        lat.sync()
+        solid = lat.solid()
+        for nm, sc in (("bx", 2e-4), ("by", 5e-5), ("bz", 1e-4)):
+            s.view(getattr(lat.s, nm), lat.N)[:] = np.where(solid, 0.0, sc * (1 + rng.random(lat.N)))
+        L.slb_sync_device(h)
+        lat.set_pops(1e-3 * rng.standard_normal(lat.N * 15))
+        L.slb_run(h, 17)
+        a = lat.pops()
+        L.measure(h)
+        f = np.stack([lat.field(n) for n in ("fx", "fy", "fz")])
+        out = (a, lat.pops(), f)
+        lat.free()
+        return out
+
+    ref, got = run("0"), run("2")
+    for r, g in zip(ref, got):
+        assert np.array_equal(r, g)
