@@ -910,6 +910,7 @@ def main():
     h, nsph, phi = build_pack(L, n, args.radius, args.phi, args.seed, args.force)
     N = n ** 3
 
+    kernel = L.slb_step_kernel(h).decode()
     L.slb_run(h, args.warmup)
     ms = L.slb_time_steps(h, args.steps)
     ms_per_step = ms / args.steps
@@ -956,10 +957,11 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / PEAK_HBM_GBS, 4),
             "traffic": traffic_bytes(n, args.precision),
-            # the library stages the k pulls in LDS for rows of 64-512 nodes
-            "kernel": ("k_step_lds (fused pull step, k pulls staged in LDS)"
-                       if args.size in (64, 128, 256, 512) and os.environ.get("SLB_STEP_LDS", "1") != "0"
-                       else "k_step (fused pull step)"),
+            # the kernel the library launches for this lattice's steps
+            "kernel": kernel + {"k_step_lds": " (fused pull step, k pulls staged in LDS)",
+                                "k_step_ldsr": " (fused pull step, k pulls staged in LDS, "
+                                               "whole rows per block)",
+                                "k_step": " (fused pull step)"}.get(kernel, ""),
             "algorithmic_bytes_per_node": bytes_per_lup,
             "device_copy_GBs": round(copy_gbs, 1),
         },

@@ -72,6 +72,13 @@ int slb_run_to_residual_aa(LBSys lb, double tol, int interval, int maxsteps, int
  * advances).  Returns milliseconds. */
 double slb_time_steps(LBSys lb, int nsteps);
 
+/* Name of the kernel that the lattice's fused steps launch, fixed when the
+ * lattice is made from its shape and precision: "k_step_lds" (rows of 64,
+ * 128, 256 or 512 nodes, k pulls staged in LDS), "k_step_ldsr" (other rows
+ * of up to 512 nodes, where staging measured faster) or "k_step".  All three
+ * give bit-identical results; SLB_STEP_LDS=0 selects k_step. */
+const char *slb_step_kernel(LBSys lb);
+
 /* Multiply every population by s (a Stokes-flow solution scales with the
  * body force). */
 void slb_scale_pops(LBSys lb, double s);

@@ -162,6 +162,7 @@ SIGNATURES = {
     "slb_insert_map": (LBObj, [LBSys, VP]),
     "slb_mp_set_adsorption": (None, [MPSys, D]), "slb_mp_track_moments": (None, [MPSys, I]),
     "slb_mp_sums": (None, [MPSys, RealP]), "slb_mp_run": (None, [MPSys, I]),
+    "slb_step_kernel": (C.c_char_p, [LBSys]),
     "slb_mp_time_steps": (D, [MPSys, I]),
     "slb_device_bytes": (C.c_size_t, [LBSys]), "slb_device_arch": (C.c_char_p, []),
     "slb_hbm_copy_test": (D, [C.c_size_t, I]),

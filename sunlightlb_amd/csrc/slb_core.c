@@ -806,6 +806,8 @@ int slb_run_to_residual_aa(LBSys lb, double tol, int interval, int maxsteps, int
   return steps;
 }
 
+const char *slb_step_kernel(LBSys lb) { return slbd_step_kernel(PRIV(lb)->dev); }
+
 double slb_time_steps(LBSys lb, int nsteps) {
   if (lb->changed_objects) syncobjs(lb);
   slbd_params P;

@@ -149,6 +149,7 @@ void slbd_vacf(slbd_lattice *L, slbd_mp *M, slbd_mp *M2, const double *psi_host,
 /* Timing support: run nsteps fused steps between HIP events on the
  * lattice stream; returns the elapsed milliseconds. */
 double slbd_time_steps(slbd_lattice *L, const slbd_params *P, int nsteps);
+const char *slbd_step_kernel(const slbd_lattice *L);
 double slbd_time_mp(slbd_lattice *L, slbd_mp *M, double rho0, double alpha,
                     double pads, int nmom, int nsteps);
 size_t slbd_bytes(const slbd_lattice *L);

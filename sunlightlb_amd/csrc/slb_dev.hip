@@ -1284,31 +1284,12 @@ extern "C" const char *slbd_arch(void) {
   return name;
 }
 
-extern "C" slbd_lattice *slbd_create(int esi, int esj, int esk, int prec) {
-  if (slbd_device_count() < 1) {
-    fprintf(stderr, "sunlightlb: no HIP device is visible; the GPU library cannot run\n");
-    exit(EXIT_FAILURE);
-  }
-  if (prec != 32 && prec != 64) prec = 64;
-  slbd_scratch_init();
-  slbd_lattice *L = (slbd_lattice *)calloc(1, sizeof(slbd_lattice));
-  L->esi = esi; L->esj = esj; L->esk = esk; L->prec = prec;
-  L->N = (long)esi * esj * esk;
-  L->idx32 = 15 * pop_nodes(L->N) <= 0x7fffffffL &&
-             !(getenv("SLB_INDEX64") && atoi(getenv("SLB_INDEX64")) != 0);
-  L->G.esi = esi; L->G.esj = esj; L->G.esk = esk; L->G.plane = esj * esk; L->G.N = L->N;
-  L->G.nt_store = getenv("SLB_NT_STORE") ? atoi(getenv("SLB_NT_STORE")) : 1;
-  L->G.nt_load = getenv("SLB_NT_LOAD") ? atoi(getenv("SLB_NT_LOAD")) : 0;
-  /* threads per block of the step kernel: 512 measured fastest at 256^3 */
-  L->G.step_tpb = 512;
-  if (getenv("SLB_STEP_BLOCK")) {
-    const int b = atoi(getenv("SLB_STEP_BLOCK"));
-    if (b == 256 || b == 512) L->G.step_tpb = b;
-  }
-  L->G.xcd_map = step_order(L->G);
-  L->sweep_nt_mid = getenv("SLB_SWEEP_NT") ? atoi(getenv("SLB_SWEEP_NT")) : 0;
+/* The step kernel of a lattice, from its shape and precision (made again
+ * when the precision changes).  k_step_lds needs whole rows per 512-thread
+ * block. */
+static void step_kernel_policy(slbd_lattice *L) {
+  const int esj = L->esj, esk = L->esk, prec = L->prec;
   {
-    /* k_step_lds needs whole rows per 512-thread block */
     /* measured: 362.5 -> 359.1 us per FP32 step at 256^3 (729 against 733 us
      * FP64), bit-identical */
     const int want = getenv("SLB_STEP_LDS") ? atoi(getenv("SLB_STEP_LDS")) : 1;
@@ -1342,6 +1323,32 @@ extern "C" slbd_lattice *slbd_create(int esi, int esj, int esk, int prec) {
       }
     }
   }
+}
+
+extern "C" slbd_lattice *slbd_create(int esi, int esj, int esk, int prec) {
+  if (slbd_device_count() < 1) {
+    fprintf(stderr, "sunlightlb: no HIP device is visible; the GPU library cannot run\n");
+    exit(EXIT_FAILURE);
+  }
+  if (prec != 32 && prec != 64) prec = 64;
+  slbd_scratch_init();
+  slbd_lattice *L = (slbd_lattice *)calloc(1, sizeof(slbd_lattice));
+  L->esi = esi; L->esj = esj; L->esk = esk; L->prec = prec;
+  L->N = (long)esi * esj * esk;
+  L->idx32 = 15 * pop_nodes(L->N) <= 0x7fffffffL &&
+             !(getenv("SLB_INDEX64") && atoi(getenv("SLB_INDEX64")) != 0);
+  L->G.esi = esi; L->G.esj = esj; L->G.esk = esk; L->G.plane = esj * esk; L->G.N = L->N;
+  L->G.nt_store = getenv("SLB_NT_STORE") ? atoi(getenv("SLB_NT_STORE")) : 1;
+  L->G.nt_load = getenv("SLB_NT_LOAD") ? atoi(getenv("SLB_NT_LOAD")) : 0;
+  /* threads per block of the step kernel: 512 measured fastest at 256^3 */
+  L->G.step_tpb = 512;
+  if (getenv("SLB_STEP_BLOCK")) {
+    const int b = atoi(getenv("SLB_STEP_BLOCK"));
+    if (b == 256 || b == 512) L->G.step_tpb = b;
+  }
+  L->G.xcd_map = step_order(L->G);
+  L->sweep_nt_mid = getenv("SLB_SWEEP_NT") ? atoi(getenv("SLB_SWEEP_NT")) : 0;
+  step_kernel_policy(L);
   {
     /* MP kernel: 256-thread blocks over (row block, plane) */
     const long nb = (long)((L->G.plane + 255) / 256) * esi;
@@ -1455,6 +1462,7 @@ extern "C" int slbd_set_precision(slbd_lattice *L, int prec, double *scratch_aos
   L->bx = L->by = L->bz = NULL;
   free_mp(L);
   L->prec = prec;
+  step_kernel_policy(L);
   const size_t pb = 15 * (size_t)pop_nodes(L->N) * tsize(prec);
   L->pops[0] = dalloc(pb);
   L->pops[1] = dalloc(pb);
@@ -2019,6 +2027,13 @@ extern "C" void slbd_step(slbd_lattice *L, const slbd_params *P, int nsteps, int
       launch_step<double>(L, A, P->moving != 0, force != 0, bf, src, dst, 0, L->esi, 0, 0, L->G);
     L->cur ^= 1;
   }
+}
+
+/* Name of the kernel that the lattice's plain steps launch. */
+extern "C" const char *slbd_step_kernel(const slbd_lattice *L) {
+  if (L->step_lds) return "k_step_lds";
+  if (L->ldsr_rpb) return "k_step_ldsr";
+  return "k_step";
 }
 
 extern "C" double slbd_time_steps(slbd_lattice *L, const slbd_params *P, int nsteps) {

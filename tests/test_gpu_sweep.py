@@ -190,3 +190,26 @@ def test_row_staged_step_bit_identical(prec, dims, monkeypatch):
     ref, got = run("0"), run("2")
     for r, g in zip(ref, got):
         assert np.array_equal(r, g)
+
+
+@pytest.mark.parametrize("dims,want", [
+    ((16, 256, 256), "k_step_lds"),      # rows of 256
+    ((32, 96, 96), "k_step_ldsr"),       # 4 rows of 96 per block, blocks on 128-byte lines
+    ((8, 200, 200), "k_step_ldsr"),      # 2 rows of 200, off the lines but cache-resident
+    ((64, 200, 200), "k_step"),          # off the lines and streamed from HBM
+    ((8, 96, 96), "k_step"),             # fewer than 2^18 nodes
+    ((4, 4, 1024), "k_step"),            # rows longer than 512
+    ((400, 7, 97), "k_step"),            # one row of 97 in a 128-thread block: 76% used
+])
+def test_step_kernel_policy(dims, want, monkeypatch):
+    """The step kernel a lattice launches (slb_step_kernel), by row length,
+    lattice size, alignment of the blocks and cache residency; SLB_STEP_LDS=0
+    always gives k_step."""
+    import sunlightlb_amd as s
+    lat = s.Lattice(*dims, precision=32)
+    assert lat.L.slb_step_kernel(lat.h).decode() == want
+    lat.free()
+    monkeypatch.setenv("SLB_STEP_LDS", "0")
+    lat = s.Lattice(*dims, precision=32)
+    assert lat.L.slb_step_kernel(lat.h).decode() == "k_step"
+    lat.free()
