@@ -375,6 +375,182 @@ k_step_ldsr(const T *__restrict__ src, T *__restrict__ dst,
   }
 }
 
+/* Moment storage.  After a collision a node's 15 populations are the
+ * rebuild of its 10 post-collision moments (rebuild_link), so a run of
+ * steps can keep the moments instead: 10 values per node in tiles of 1024
+ * nodes x 10 runs, read and written instead of 15 populations.  The step
+ * rebuilds each pulled a_l(x - c_l) from the source node's moments (links
+ * grouped by the source node: own, x -+ 1, y -+ 1 and the four diagonal
+ * sources), the k-shifted links through LDS as in k_step_lds, and bounced
+ * links from the node's own moments; the collision is collide_node's.  The
+ * first step of a run reads populations (IN_POPS), the last writes them
+ * (OUT_POPS).  Every pulled value equals the population the plain step
+ * would have stored and loaded, so the populations after the run are
+ * bit-identical to k_step's.  Needs rows of 64-512 nodes dividing 512, no
+ * skip_solid (a skipped solid node keeps 15 independent populations) and no
+ * force accumulation. */
+template <typename I>
+__host__ __device__ __forceinline__ I midx_t(int m, I x) {
+  return ((x >> POP_SHIFT) * I(10) + I(m)) * I(POP_TILE) + (x & I(POP_TILE - 1));
+}
+
+template <typename T, typename I>
+__device__ __forceinline__ Mom<T> load_mom(const T *__restrict__ M, I y) {
+  Mom<T> m;
+  m.drho = M[midx_t<I>(0, y)];
+  m.jx = M[midx_t<I>(1, y)];
+  m.jy = M[midx_t<I>(2, y)];
+  m.jz = M[midx_t<I>(3, y)];
+  m.sxx = M[midx_t<I>(4, y)];
+  m.syy = M[midx_t<I>(5, y)];
+  m.szz = M[midx_t<I>(6, y)];
+  m.sxy = M[midx_t<I>(7, y)];
+  m.sxz = M[midx_t<I>(8, y)];
+  m.syz = M[midx_t<I>(9, y)];
+  return m;
+}
+
+/* the moments an axis link's rebuild reads (drho, the trace, one j) */
+template <typename T, typename I, int AX>
+__device__ __forceinline__ Mom<T> load_mom_axis(const T *__restrict__ M, I y) {
+  Mom<T> m;
+  m.drho = M[midx_t<I>(0, y)];
+  m.jx = (AX == 0) ? M[midx_t<I>(1, y)] : T(0);
+  m.jy = (AX == 1) ? M[midx_t<I>(2, y)] : T(0);
+  m.jz = (AX == 2) ? M[midx_t<I>(3, y)] : T(0);
+  m.sxx = M[midx_t<I>(4, y)];
+  m.syy = M[midx_t<I>(5, y)];
+  m.szz = M[midx_t<I>(6, y)];
+  m.sxy = m.sxz = m.syz = T(0);
+  return m;
+}
+
+template <typename T, bool MOVING, int BF, typename I, bool IN_POPS, bool OUT_POPS>
+__global__ void __launch_bounds__(512)
+k_step_mom(const T *__restrict__ srcP, const T *__restrict__ srcM, T *__restrict__ dstP,
+           T *__restrict__ dstM, const uint16_t *__restrict__ flags,
+           const double *__restrict__ ux, const double *__restrict__ uy,
+           const double *__restrict__ uz,
+           const T *__restrict__ bx, const T *__restrict__ by, const T *__restrict__ bz,
+           Geom G, PhysArgs P) {
+  __shared__ T row[10][512];
+  int blk_x, blk_y;
+  step_block(G, blk_x, blk_y);
+  const int t = blk_x * 512 + threadIdx.x;          /* plane % 512 == 0: all in range */
+  const int i = blk_y;
+  const int kmask = G.esk - 1;
+  const int j = t / G.esk;
+  const int k = t & kmask;
+  const I x = (I)i * (I)G.plane + (I)t;
+  const AxisOffT<I> oi = axis_off_t<I>(i, G.esi, (I)G.plane);
+  const AxisOffT<I> oj = axis_off_t<I>(j, G.esj, (I)G.esk);
+  const AxisOffT<I> ok = axis_off_t<I>(k, G.esk, (I)1);
+  const AxisOffT<I> o0 = {0, 0};
+  const unsigned fl = flags[x];
+  const bool solid = (fl & FLAG_SOLID) != 0;
+  T v[15];
+  T own[15];
+  if constexpr (IN_POPS) {
+    Links<5, 15>::run([&](auto LL) {
+      constexpr int l = decltype(LL)::value;
+      row[l - 5][threadIdx.x] = srcP[pidx_t<I>(l, x + nb_off<l, -1>(oi, oj, o0))];
+    });
+    v[0] = srcP[pidx_t<I>(0, x)];
+    Links<1, 5>::run([&](auto LL) {
+      constexpr int l = decltype(LL)::value;
+      const bool bb = (fl >> (l - 1)) & 1u;
+      v[l] = srcP[bb ? pidx_t<I>(OPP(l), x) : pidx_t<I>(l, x + nb_off<l, -1>(oi, oj, ok))];
+    });
+    Links<5, 15>::run([&](auto LL) {
+      constexpr int l = decltype(LL)::value;
+      own[l] = ((fl >> (l - 1)) & 1u) ? srcP[pidx_t<I>(OPP(l), x)] : T(0);
+    });
+  } else {
+    const Mom<T> mo = load_mom<T, I>(srcM, x);
+    /* diagonal sources (cx, cy) = (+1,+1): links 7, 11; (-1,+1): 8, 12;
+     * (+1,-1): 9, 13; (-1,-1): 10, 14 -- at the node's own k */
+    {
+      const Mom<T> m = load_mom<T, I>(srcM, x + nb_off<7, -1>(oi, oj, o0));
+      row[7 - 5][threadIdx.x] = rebuild_link<7>(m);
+      row[11 - 5][threadIdx.x] = rebuild_link<11>(m);
+    }
+    {
+      const Mom<T> m = load_mom<T, I>(srcM, x + nb_off<8, -1>(oi, oj, o0));
+      row[8 - 5][threadIdx.x] = rebuild_link<8>(m);
+      row[12 - 5][threadIdx.x] = rebuild_link<12>(m);
+    }
+    {
+      const Mom<T> m = load_mom<T, I>(srcM, x + nb_off<9, -1>(oi, oj, o0));
+      row[9 - 5][threadIdx.x] = rebuild_link<9>(m);
+      row[13 - 5][threadIdx.x] = rebuild_link<13>(m);
+    }
+    {
+      const Mom<T> m = load_mom<T, I>(srcM, x + nb_off<10, -1>(oi, oj, o0));
+      row[10 - 5][threadIdx.x] = rebuild_link<10>(m);
+      row[14 - 5][threadIdx.x] = rebuild_link<14>(m);
+    }
+    row[5 - 5][threadIdx.x] = rebuild_link<5>(mo);
+    row[6 - 5][threadIdx.x] = rebuild_link<6>(mo);
+    v[0] = rebuild_link<0>(mo);
+    {
+      const bool b1 = fl & 1u, b2 = (fl >> 1) & 1u, b3 = (fl >> 2) & 1u, b4 = (fl >> 3) & 1u;
+      const Mom<T> m1 = load_mom_axis<T, I, 0>(srcM, x + nb_off<1, -1>(oi, oj, ok));
+      const Mom<T> m2 = load_mom_axis<T, I, 0>(srcM, x + nb_off<2, -1>(oi, oj, ok));
+      const Mom<T> m3 = load_mom_axis<T, I, 1>(srcM, x + nb_off<3, -1>(oi, oj, ok));
+      const Mom<T> m4 = load_mom_axis<T, I, 1>(srcM, x + nb_off<4, -1>(oi, oj, ok));
+      v[1] = b1 ? rebuild_link<2>(mo) : rebuild_link<1>(m1);
+      v[2] = b2 ? rebuild_link<1>(mo) : rebuild_link<2>(m2);
+      v[3] = b3 ? rebuild_link<4>(mo) : rebuild_link<3>(m3);
+      v[4] = b4 ? rebuild_link<3>(mo) : rebuild_link<4>(m4);
+    }
+    Links<5, 15>::run([&](auto LL) {
+      constexpr int l = decltype(LL)::value;
+      own[l] = ((fl >> (l - 1)) & 1u) ? rebuild_link<OPP(l)>(mo) : T(0);
+    });
+  }
+  __syncthreads();
+  const int rbase = threadIdx.x - k;
+  Links<5, 15>::run([&](auto LL) {
+    constexpr int l = decltype(LL)::value;
+    const T pulled = row[l - 5][rbase + ((k - CZ(l)) & kmask)];
+    v[l] = ((fl >> (l - 1)) & 1u) ? own[l] : pulled;
+  });
+  if (MOVING && (fl & FLAG_LINKS)) {
+    Links<1, 15>::run([&](auto LL) {
+      constexpr int l = decltype(LL)::value;
+      if ((fl >> (l - 1)) & 1u) {
+        const I s = solid ? x : x + nb_off<l, -1>(oi, oj, ok);
+        const double cu =
+            (double)WL(l) * ((CX(l) ? CX(l) * ux[s] : 0.0) + (CY(l) ? CY(l) * uy[s] : 0.0) +
+                             (CZ(l) ? CZ(l) * uz[s] : 0.0));
+        v[l] = (T)((double)v[l] + cu);
+      }
+    });
+  }
+  T jbx = T(0), jby = T(0), jbz = T(0);
+  if (BF != 0 && !solid) {
+    jbx = T(P.bfx); jby = T(P.bfy); jbz = T(P.bfz);
+    if (BF == 2) { jbx += bx[x]; jby += by[x]; jbz += bz[x]; }
+  }
+  const Mom<T> mn = collide_moments<T>(v, jbx, jby, jbz, P);
+  if constexpr (OUT_POPS) {
+    rebuild_all<T>(mn, v);
+#pragma unroll
+    for (int l = 0; l < 15; l++) __builtin_nontemporal_store(v[l], &dstP[pidx_t<I>(l, x)]);
+  } else {
+    __builtin_nontemporal_store(mn.drho, &dstM[midx_t<I>(0, x)]);
+    __builtin_nontemporal_store(mn.jx, &dstM[midx_t<I>(1, x)]);
+    __builtin_nontemporal_store(mn.jy, &dstM[midx_t<I>(2, x)]);
+    __builtin_nontemporal_store(mn.jz, &dstM[midx_t<I>(3, x)]);
+    __builtin_nontemporal_store(mn.sxx, &dstM[midx_t<I>(4, x)]);
+    __builtin_nontemporal_store(mn.syy, &dstM[midx_t<I>(5, x)]);
+    __builtin_nontemporal_store(mn.szz, &dstM[midx_t<I>(6, x)]);
+    __builtin_nontemporal_store(mn.sxy, &dstM[midx_t<I>(7, x)]);
+    __builtin_nontemporal_store(mn.sxz, &dstM[midx_t<I>(8, x)]);
+    __builtin_nontemporal_store(mn.syz, &dstM[midx_t<I>(9, x)]);
+  }
+}
+
 /* ---------------------------------------------------------------------- */
 /* Separate phases (bbl, propagate, collide) on the live buffer.           */
 
@@ -1220,6 +1396,7 @@ struct slbd_lattice {
   int sweep_nt_mid;         /* two-step sweep: non-temporal window stores */
   int step_lds;             /* step with the k pulls staged in LDS (k_step_lds) */
   int ldsr_rpb, ldsr_bs;    /* k_step_ldsr: rows per block, threads per block (0: off) */
+  void *mom[2];             /* moment storage of a run (k_step_mom), or NULL */
   void *sph_buf;            /* sphere images of a pack being placed (grow-only) */
   size_t sph_cap;
 };
@@ -1392,7 +1569,7 @@ extern "C" void slbd_destroy(slbd_lattice *L) {
   HIPCHECK(hipStreamSynchronize(L->stream));
   void *ps[] = {L->pops[0], L->pops[1], L->flags, L->objidx, L->mask, L->ux, L->uy, L->uz,
                 L->fx, L->fy, L->fz, L->bx, L->by, L->bz, L->part, L->dout, L->psi,
-                L->tab, L->acc, L->uprev, L->sph_buf};
+                L->tab, L->acc, L->uprev, L->sph_buf, L->mom[0], L->mom[1]};
   for (size_t q = 0; q < sizeof ps / sizeof ps[0]; q++)
     if (ps[q]) HIPCHECK(hipFree(ps[q]));
   free_mp(L);
@@ -1461,6 +1638,8 @@ extern "C" int slbd_set_precision(slbd_lattice *L, int prec, double *scratch_aos
   if (L->bx) { HIPCHECK(hipFree(L->bx)); HIPCHECK(hipFree(L->by)); HIPCHECK(hipFree(L->bz)); }
   L->bx = L->by = L->bz = NULL;
   free_mp(L);
+  for (int q = 0; q < 2; q++)
+    if (L->mom[q]) { HIPCHECK(hipFree(L->mom[q])); L->mom[q] = NULL; }
   L->prec = prec;
   step_kernel_policy(L);
   const size_t pb = 15 * (size_t)pop_nodes(L->N) * tsize(prec);
@@ -2007,11 +2186,61 @@ static void sweep2(slbd_lattice *L, const PhysArgs &A, bool moving, int bf, int 
   }
 }
 
+/* Runs of steps on moment storage (k_step_mom): SLB_STEP_MOM=1, read per
+ * call; runs of 2 steps or more without force accumulation, on lattices the
+ * staged step covers, with 32-bit indices and skip_solid off. */
+static bool mom_run_ok(const slbd_lattice *L, const slbd_params *P, int nsteps, int force) {
+  const char *e = getenv("SLB_STEP_MOM");
+  if (!(e && atoi(e) == 1)) return false;
+  return !force && nsteps >= 2 && L->step_lds && L->idx32 && !P->skip_solid;
+}
+
+template <typename T, bool IN_POPS, bool OUT_POPS>
+static void launch_mom(slbd_lattice *L, const PhysArgs &A, bool moving, int bf,
+                       const T *srcP, const T *srcM, T *dstP, T *dstM) {
+  const dim3 g2((unsigned)(L->G.plane / 512), (unsigned)L->esi);
+  const T *bx = (const T *)L->bx, *by = (const T *)L->by, *bz = (const T *)L->bz;
+#define SLB_MOM_CASE(M, B)                                                                    \
+  if (moving == M && bf == B) {                                                               \
+    hipLaunchKernelGGL((k_step_mom<T, M, B, int, IN_POPS, OUT_POPS>), g2, dim3(512), 0,      \
+                       L->stream, srcP, srcM, dstP, dstM, L->flags, L->ux, L->uy, L->uz, bx, \
+                       by, bz, L->G, A);                                                      \
+    LAUNCHCHECK();                                                                            \
+    return;                                                                                   \
+  }
+  SLB_MOM_CASE(false, 0) SLB_MOM_CASE(false, 1) SLB_MOM_CASE(false, 2)
+  SLB_MOM_CASE(true, 0) SLB_MOM_CASE(true, 1) SLB_MOM_CASE(true, 2)
+#undef SLB_MOM_CASE
+}
+
+template <typename T>
+static void mom_run(slbd_lattice *L, const PhysArgs &A, bool moving, int bf, int nsteps) {
+  const size_t mb = 10 * (size_t)pop_nodes(L->N) * sizeof(T);
+  for (int q = 0; q < 2; q++)
+    if (!L->mom[q]) L->mom[q] = dalloc(mb);
+  T *M[2] = {(T *)L->mom[0], (T *)L->mom[1]};
+  const T *P0 = (const T *)L->pops[L->cur];
+  T *P1 = (T *)L->pops[L->cur ^ 1];
+  launch_mom<T, true, false>(L, A, moving, bf, P0, nullptr, nullptr, M[0]);
+  int a = 0;
+  for (int s = 1; s < nsteps - 1; s++) {
+    launch_mom<T, false, false>(L, A, moving, bf, nullptr, M[a], nullptr, M[a ^ 1]);
+    a ^= 1;
+  }
+  launch_mom<T, false, true>(L, A, moving, bf, nullptr, M[a], P1, nullptr);
+  L->cur ^= 1;
+}
+
 extern "C" void slbd_step(slbd_lattice *L, const slbd_params *P, int nsteps, int force) {
   const PhysArgs A = phys(P);
   const int bf = bf_mode(L, P);
   if (force) ensure_f(L);
   const int sw = force ? 0 : sweep_planes(L);
+  if (sw == 0 && mom_run_ok(L, P, nsteps, force)) {
+    if (L->prec == 32) mom_run<float>(L, A, P->moving != 0, bf, nsteps);
+    else mom_run<double>(L, A, P->moving != 0, bf, nsteps);
+    return;
+  }
   int s = 0;
   if (sw > 0) {
     for (; s + 2 <= nsteps; s += 2) {

@@ -179,9 +179,17 @@ struct PhysArgs {
   double bfx, bfy, bfz;
 };
 
+/* Post-collision moments of a node: density deviation, momentum (with the
+ * body force) and the relaxed traceless-shifted stress.  The 15
+ * post-collision populations are rebuilt from these 10 numbers alone. */
 template <typename T>
-__device__ __forceinline__ void collide_node(T v[15], T jbx, T jby, T jbz,
-                                             const PhysArgs &P) {
+struct Mom {
+  T drho, jx, jy, jz, sxx, syy, szz, sxy, sxz, syz;
+};
+
+template <typename T>
+__device__ __forceinline__ Mom<T> collide_moments(const T v[15], T jbx, T jby, T jbz,
+                                                  const PhysArgs &P) {
   const T ax = v[1] + v[2], dx = v[1] - v[2];
   const T ay = v[3] + v[4], dy = v[3] - v[4];
   const T az = v[5] + v[6], dz = v[5] - v[6];
@@ -199,44 +207,89 @@ __device__ __forceinline__ void collide_node(T v[15], T jbx, T jby, T jbz,
   const T Dxz = (p7 - p8) + (p9 - p10);
   const T Dyz = (p7 + p8) - (p9 + p10);
   const T a0a = v[0] + ax + ay + az;
-  const T drho = a0a + D;
-  const T jx = dx + Dx + jbx;
-  const T jy = dy + Dy + jby;
-  const T jz = dz + Dz + jbz;
+  Mom<T> m;
+  m.drho = a0a + D;
+  m.jx = dx + Dx + jbx;
+  m.jy = dy + Dy + jby;
+  m.jz = dz + Dz + jbz;
   /* traceless-shifted stress s = Pi - drho/3 I (src/lbkernel.c:821-823) */
-  const T sh = T(2.0 / 3.0) * drho - a0a;
+  const T sh = T(2.0 / 3.0) * m.drho - a0a;
   T sxx = sh + ax, syy = sh + ay, szz = sh + az;
   T sxy = Dxy, sxz = Dxz, syz = Dyz;
   const T cw = T(P.comega);
   if (P.stokes) {
     sxx *= cw; syy *= cw; szz *= cw; sxy *= cw; sxz *= cw; syz *= cw;
   } else {
-    const T o = T(P.omega) / (T(P.rho0) + drho);
-    sxx = cw * sxx + o * jx * jx;
-    syy = cw * syy + o * jy * jy;
-    szz = cw * szz + o * jz * jz;
-    sxy = cw * sxy + o * jx * jy;
-    sxz = cw * sxz + o * jx * jz;
-    syz = cw * syz + o * jy * jz;
+    const T o = T(P.omega) / (T(P.rho0) + m.drho);
+    sxx = cw * sxx + o * m.jx * m.jx;
+    syy = cw * syy + o * m.jy * m.jy;
+    szz = cw * szz + o * m.jz * m.jz;
+    sxy = cw * sxy + o * m.jx * m.jy;
+    sxz = cw * sxz + o * m.jx * m.jz;
+    syz = cw * syz + o * m.jy * m.jz;
   }
-  /* rebuild a_l = (w_l/72)[drho + 3 c.j + 4.5 (cc:s - tr s/3)] */
-  const T r72 = drho * T(1.0 / 72.0);
-  const T r9 = T(8.0) * r72;
-  const T t3 = (sxx + syy + szz) * T(1.0 / 3.0);
-  const T third = T(1.0 / 3.0), half = T(0.5), eighth = T(0.125), d24 = T(1.0 / 24.0);
-  v[0] = T(2.0) * r9 - t3;
-  T q, g;
-  q = r9 + half * (sxx - t3); g = third * jx; v[1] = q + g; v[2] = q - g;
-  q = r9 + half * (syy - t3); g = third * jy; v[3] = q + g; v[4] = q - g;
-  q = r9 + half * (szz - t3); g = third * jz; v[5] = q + g; v[6] = q - g;
-  q = r72 + eighth * (t3 + sxy + sxz + syz); g = d24 * (jx + jy + jz);
-  v[7] = q + g; v[14] = q - g;
-  q = r72 + eighth * (t3 - sxy - sxz + syz); g = d24 * (-jx + jy + jz);
-  v[8] = q + g; v[13] = q - g;
-  q = r72 + eighth * (t3 - sxy + sxz - syz); g = d24 * (jx - jy + jz);
-  v[9] = q + g; v[12] = q - g;
-  q = r72 + eighth * (t3 + sxy - sxz - syz); g = d24 * (-jx - jy + jz);
-  v[10] = q + g; v[11] = q - g;
+  m.sxx = sxx; m.syy = syy; m.szz = szz; m.sxy = sxy; m.sxz = sxz; m.syz = syz;
+  return m;
+}
+
+/* Post-collision population of link l rebuilt from the moments,
+ * a_l = (w_l/72)[drho + 3 c.j + 4.5 (cc:s - tr s/3)] (src/lbkernel.c:837-865),
+ * with no contraction into fused multiply-adds, so that the rebuild gives the
+ * same value wherever it is evaluated (the step that stores populations and
+ * the step that pulls them from stored moments). */
+template <int l, typename T>
+__device__ __forceinline__ T rebuild_link(const Mom<T> &m) {
+#pragma clang fp contract(off)
+  const T r72 = m.drho * T(1.0 / 72.0);
+  const T t3 = (m.sxx + m.syy + m.szz) * T(1.0 / 3.0);
+  if constexpr (l == 0) {
+    const T r9 = T(8.0) * r72;
+    return T(2.0) * r9 - t3;
+  } else if constexpr (l <= 6) {
+    const T r9 = T(8.0) * r72;
+    const T sd = (l <= 2) ? m.sxx : (l <= 4) ? m.syy : m.szz;
+    const T jd = (l <= 2) ? m.jx : (l <= 4) ? m.jy : m.jz;
+    const T q = r9 + T(0.5) * (sd - t3);
+    const T g = T(1.0 / 3.0) * jd;
+    return (l & 1) ? q + g : q - g;
+  } else if constexpr (l == 7 || l == 14) {
+    const T q = r72 + T(0.125) * (t3 + m.sxy + m.sxz + m.syz);
+    const T g = T(1.0 / 24.0) * (m.jx + m.jy + m.jz);
+    return (l == 7) ? q + g : q - g;
+  } else if constexpr (l == 8 || l == 13) {
+    const T q = r72 + T(0.125) * (t3 - m.sxy - m.sxz + m.syz);
+    const T g = T(1.0 / 24.0) * (-m.jx + m.jy + m.jz);
+    return (l == 8) ? q + g : q - g;
+  } else if constexpr (l == 9 || l == 12) {
+    const T q = r72 + T(0.125) * (t3 - m.sxy + m.sxz - m.syz);
+    const T g = T(1.0 / 24.0) * (m.jx - m.jy + m.jz);
+    return (l == 9) ? q + g : q - g;
+  } else {
+    const T q = r72 + T(0.125) * (t3 + m.sxy - m.sxz - m.syz);
+    const T g = T(1.0 / 24.0) * (-m.jx - m.jy + m.jz);
+    return (l == 10) ? q + g : q - g;
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ void rebuild_all(const Mom<T> &m, T v[15]) {
+  v[0] = rebuild_link<0>(m);
+  v[1] = rebuild_link<1>(m); v[2] = rebuild_link<2>(m);
+  v[3] = rebuild_link<3>(m); v[4] = rebuild_link<4>(m);
+  v[5] = rebuild_link<5>(m); v[6] = rebuild_link<6>(m);
+  v[7] = rebuild_link<7>(m); v[8] = rebuild_link<8>(m);
+  v[9] = rebuild_link<9>(m); v[10] = rebuild_link<10>(m);
+  v[11] = rebuild_link<11>(m); v[12] = rebuild_link<12>(m);
+  v[13] = rebuild_link<13>(m); v[14] = rebuild_link<14>(m);
+}
+
+/* Collision of one node (src/lbkernel.c:797-866): moments, relaxation,
+ * rebuild of the 15 populations in place. */
+template <typename T>
+__device__ __forceinline__ void collide_node(T v[15], T jbx, T jby, T jbz,
+                                             const PhysArgs &P) {
+  const Mom<T> m = collide_moments<T>(v, jbx, jby, jbz, P);
+  rebuild_all<T>(m, v);
 }
 
 /* Wall velocity of solid nodes from the object table, src/lbobjs.c:565-589:
