@@ -148,6 +148,36 @@ def test_pack256_fp32_permeability_within_1e5_of_fp64():
     assert 1.0 < k64 < 10.0            # lattice units; a 0.35 pack of R 8
 
 
+@pytest.mark.parametrize("prec", [32, 64])
+def test_pack96_row_staged_step_bit_identical(prec, monkeypatch):
+    """A 96^3 pack (rows of 96 nodes: the staged step with 4 whole rows per
+    block, k_step_ldsr, chosen by default) against the plain k_step on the
+    same pack: populations after 60 steps and the forces and mean velocity of
+    measure() bit-identical."""
+    L = lib()
+
+    def run(lds):
+        monkeypatch.setenv("SLB_STEP_LDS", lds)
+        monkeypatch.setenv("SLB_PRECISION", str(prec))
+        h, nsph, phi = bench.build_pack(L, 96, 3.0, 0.35, SEED, 1e-6)
+        kern = L.slb_step_kernel(h).decode()
+        L.slb_run(h, 60)
+        from sunlightlb_amd import Lattice
+        lat = Lattice.wrap(h)
+        a = lat.pops()
+        L.measure(h)
+        out = (kern, a, np.stack([lat.field(n) for n in ("fx", "fy", "fz")]),
+               (h.contents.vmeanx, h.contents.vmeany, h.contents.vmeanz))
+        L.slb_destroy(h)
+        return out
+
+    ref, got = run("0"), run("1")
+    assert ref[0] == "k_step" and got[0] == "k_step_ldsr"
+    assert np.array_equal(ref[1], got[1])
+    assert np.array_equal(ref[2], got[2])
+    assert ref[3] == got[3]
+
+
 # ---------------------------------------------------------- 128^3 MP workload
 
 def mp_setup(L, prec, pads=0.01, peclet=10.0):
