@@ -1479,7 +1479,8 @@ static void step_kernel_policy(slbd_lattice *L) {
      * more, and where either every block starts on a 128-byte line or both
      * population buffers fit the last-level cache.  Measured FP32 against
      * k_step: 80^3 13.2/15.0 us, 96^3 20.1/23.5, 100^3 26.4/27.2, 120^3
-     * 42.3/43.6, 160^3 94.0/99.0 (FP64 96^3 38.6/43.2, 160^3 185/188); but
+     * 42.3/43.6, 160^3 94.0/99.0, 192^3 154.6/154.3, 224^3 241.4/259.2 (FP64
+     * 96^3 38.6/43.2, 160^3 185/188); but
      * rows whose blocks start off the line boundary, streamed from HBM, are
      * slower (200^3 212/181, 300^3 724/646 us), as are lattices too small
      * to fill the GPU (32^3 4.7/4.5, 48^3 7.0/6.1 us).  SLB_STEP_LDS=2 uses
